@@ -1,0 +1,214 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+only as the checker (or the timed CPU baseline).  The product library never
+imports it.
+
+* ``liboracle.so``: the C restatement in pentad_oracle.c (Thomas recurrence of
+  src/cpu/singlenode/pentad_orig_cpu.cpp:73-122 with the routing of :158-212,
+  a long-double variant, banded and dense LU with partial pivoting, residuals).
+* ``_ref/libref_avx.so`` / ``_ref/libref_orig.so``: the reference's own CPU
+  solvers (src/cpu/singlenode/pentad_cpu.cpp, pentad_orig_cpu.cpp) compiled
+  from its sources by oracle/Makefile, when they were built.
+* ``read_fixture``: reader of the reference fixture text format
+  (include/util/mesh.hpp:50-80).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_AVX = os.path.join(HERE, "_ref", "libref_avx.so")
+REF_ORIG = os.path.join(HERE, "_ref", "libref_orig.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_lib = None
+_ref = {}
+
+
+def build() -> None:
+    """Compile the oracle (and _ref when the reference sources are present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        l = ctypes.CDLL(LIB)
+        dbl = [_P] * 6 + [_P, _I, _I]
+        l.po_gpsv_batch_d.argtypes = dbl
+        l.po_gpsv_batch_s.argtypes = dbl
+        l.po_gpsv_batch_d.restype = _I
+        l.po_gpsv_batch_s.restype = _I
+        l.po_gpsv_batch_ld.argtypes = [_P] * 7 + [_P, _I, _I]
+        l.po_gpsv_batch_ld.restype = _I
+        l.po_gpsv_batch_pivot_d.argtypes = [_P] * 7 + [_P, _I, _I, _I]
+        l.po_gpsv_batch_pivot_d.restype = _I
+        l.po_residual_d.argtypes = [_P] * 7 + [_P, _I, _I]
+        l.po_residual_d.restype = ctypes.c_double
+        l.po_varlen_d.argtypes = [_P] * 6 + [_P, ctypes.c_int64]
+        l.po_varlen_d.restype = _I
+        l.po_varlen_ld.argtypes = [_P] * 7 + [_P, ctypes.c_int64]
+        l.po_varlen_ld.restype = _I
+        _lib = l
+    return _lib
+
+
+def _c(a: np.ndarray, dtype) -> np.ndarray:
+    if a.dtype != dtype or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError(f"expected C-contiguous {dtype}")
+    return a
+
+
+def _dims(dims: Sequence[int]):
+    return (ctypes.c_int * len(dims))(*[int(v) for v in dims])
+
+
+def _pp(*arrs):
+    return [a.ctypes.data_as(_P) for a in arrs]
+
+
+def gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim) -> None:
+    """In-place Thomas solve along solvedim (float64 or float32 arrays)."""
+    dt = x.dtype
+    for a in (ds, dl, d, du, dw, x):
+        _c(a, dt)
+    fn = lib().po_gpsv_batch_d if dt == np.float64 else lib().po_gpsv_batch_s
+    rc = fn(*_pp(ds, dl, d, du, dw, x), _dims(dims), len(dims), int(solvedim))
+    if rc != 0:
+        raise ValueError(f"oracle rc={rc}")
+
+
+def solve(ds, dl, d, du, dw, b, dims, solvedim) -> np.ndarray:
+    x = b.copy()
+    gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim)
+    return x
+
+
+def solve_ld(ds, dl, d, du, dw, b, dims, solvedim) -> np.ndarray:
+    """Long-double Thomas (float64 in/out): forward-error reference."""
+    arrs = [_c(np.ascontiguousarray(a, np.float64), np.float64)
+            for a in (ds, dl, d, du, dw, b)]
+    out = np.empty_like(arrs[5])
+    rc = lib().po_gpsv_batch_ld(*_pp(*arrs, out), _dims(dims), len(dims), int(solvedim))
+    if rc != 0:
+        raise ValueError(f"oracle rc={rc}")
+    return out
+
+
+def solve_pivot(ds, dl, d, du, dw, b, dims, solvedim, method: str = "banded") -> np.ndarray:
+    """Partial-pivoting solve: method 'banded' (gbsv style) or 'dense'."""
+    arrs = [_c(np.ascontiguousarray(a, np.float64), np.float64)
+            for a in (ds, dl, d, du, dw, b)]
+    out = np.empty_like(arrs[5])
+    rc = lib().po_gpsv_batch_pivot_d(*_pp(*arrs, out), _dims(dims), len(dims),
+                                     int(solvedim), 0 if method == "banded" else 1)
+    if rc != 0:
+        raise ValueError(f"oracle pivot rc={rc}")
+    return out
+
+
+def residual(ds, dl, d, du, dw, x, b, dims, solvedim) -> float:
+    """max |A x - b| (long double accumulation)."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (ds, dl, d, du, dw, x, b)]
+    return float(lib().po_residual_d(*_pp(*arrs), _dims(dims), len(dims), int(solvedim)))
+
+
+def varlen(ds, dl, d, du, dw, b, offsets, use_ld: bool = False) -> np.ndarray:
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (ds, dl, d, du, dw, b)]
+    off = np.ascontiguousarray(offsets, np.int64)
+    nsys = len(off) - 1
+    if use_ld:
+        out = np.empty_like(arrs[5])
+        rc = lib().po_varlen_ld(*_pp(*arrs, out), off.ctypes.data_as(_P), nsys)
+    else:
+        out = arrs[5].copy()
+        rc = lib().po_varlen_d(*_pp(*arrs[:5], out), off.ctypes.data_as(_P), nsys)
+    if rc != 0:
+        raise ValueError(f"oracle varlen rc={rc}")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# reference CPU solver built from its own sources (oracle/_ref)
+# ---------------------------------------------------------------------------
+def ref_available(kind: str = "avx") -> bool:
+    return os.path.exists(REF_AVX if kind == "avx" else REF_ORIG)
+
+
+def ref_lib(kind: str = "avx") -> ctypes.CDLL:
+    if kind not in _ref:
+        l = ctypes.CDLL(REF_AVX if kind == "avx" else REF_ORIG)
+        for nm in ("ref_gpsv_batch_d", "ref_gpsv_batch_s"):
+            fn = getattr(l, nm)
+            fn.argtypes = [_P] * 6 + [_P, _I, _I]
+            fn.restype = _I
+        _ref[kind] = l
+    return _ref[kind]
+
+
+def ref_gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, kind: str = "avx") -> None:
+    """In-place solve by the reference's own CPU library.
+
+    The AVX path needs dims[0] to be a multiple of its SIMD width for 2-D/3-D
+    x-solves (reference pentad_cpu.cpp:611-613) and len <= 1024 (:11).
+    """
+    dt = x.dtype
+    for a in (ds, dl, d, du, dw, x):
+        _c(a, dt)
+    l = ref_lib(kind)
+    fn = l.ref_gpsv_batch_d if dt == np.float64 else l.ref_gpsv_batch_s
+    fn(*_pp(ds, dl, d, du, dw, x), _dims(dims), len(dims), int(solvedim))
+
+
+# ---------------------------------------------------------------------------
+# fixture reader (reference include/util/mesh.hpp:50-80)
+# ---------------------------------------------------------------------------
+def read_fixture(path: str, dtype=np.float64) -> Dict[str, object]:
+    """Returns dims, solvedim, ds, dl, d, du, dw, x (RHS) and u (solution).
+
+    Values are parsed as float64 and cast to dtype; for float32 the expected
+    solution is the 6-decimal FP32 line (the reference skips the FP64 line,
+    mesh.hpp:72-79).
+    """
+    with open(path) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.strip()]
+    ndims, solvedim = (int(v) for v in lines[0].split())
+    dims = [int(v) for v in lines[1].split()]
+    assert len(dims) == ndims
+    n = int(np.prod(dims))
+
+    def arr(i):
+        a = np.array(lines[i].split(), dtype=np.float64)
+        assert a.size == n, (path, i, a.size, n)
+        return a.astype(dtype)
+
+    keys = ["ds", "dl", "d", "du", "dw", "x"]
+    out = {k: arr(2 + j) for j, k in enumerate(keys)}
+    out["u"] = arr(8) if dtype == np.float64 else arr(9)
+    out["dims"] = dims
+    out["solvedim"] = solvedim
+    return out
+
+
+def allclose_reference_rule(expected, actual, dtype) -> bool:
+    """Reference test rule (test/catch_utils.hpp:19-25): every element within
+    rel 1e-13 or abs 1e-15 (FP64), rel 1e-5 or abs 1e-6 (FP32)."""
+    e = np.asarray(expected, np.float64)
+    a = np.asarray(actual, np.float64)
+    if dtype == np.float64:
+        rel, ab = 1e-13, 1e-15
+    else:
+        rel, ab = 1e-5, 1e-6
+    diff = np.abs(e - a)
+    ok = (diff <= ab) | (diff <= rel * np.maximum(np.abs(e), np.abs(a)))
+    return bool(np.all(ok))

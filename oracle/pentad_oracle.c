@@ -1,0 +1,446 @@
+/*
+ * pentad_oracle.c -- CPU restatement of the reference pentadiagonal solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library: it is built into oracle/liboracle.so and loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The product
+ * library (pentadiagonal-solver_amd/) never links, loads or calls it.
+ *
+ * What each function restates (paths relative to the reference repository):
+ *
+ *   po_thomas_{d,s}     banded LU without pivoting ("pentadiagonal Thomas"),
+ *                       same operation order as
+ *                       src/cpu/singlenode/pentad_orig_cpu.cpp:73-122
+ *                       (strided form; stride 1 is the x form of :9-56).
+ *   po_gpsv_batch_{d,s} direction routing and line decomposition of
+ *                       pentad_orig_cpu.cpp:124-212: solvedim 0 -> contiguous
+ *                       lines, solvedim ndim-1 -> stride prod(dims[:-1]),
+ *                       otherwise n_in = prod(dims[:sd]) lines of stride n_in
+ *                       repeated n_out = prod(dims[sd+1:]) times.
+ *   po_thomas_ld        the same recurrence in long double: the independent
+ *                       high-precision reference used for forward errors
+ *                       (role of apps/fp32_accuracy_cuda.cu:48-73).
+ *   po_banded_lu_d      Gaussian elimination with partial pivoting on the
+ *                       (2,2)-banded matrix, LAPACK gbsv style: the method
+ *                       test/gen_testcases.py:27-33 used to write the golden
+ *                       fixtures (scipy solve_banded).
+ *   po_dense_lu_d       dense LU with partial pivoting (the "dense FP64 LU
+ *                       oracle" of the 64^3 CPU configuration).
+ *   po_residual_d       max-norm backward residual |A x - b| per batch, the
+ *                       check of apps/verify_strided_cuda.cu:143-173.
+ *
+ * Out-of-range couplings (ds of rows 0-1, dl of row 0, du of row N-1 and dw
+ * of rows N-2..N-1) are ignored, as in the reference: the Thomas recurrence
+ * never lets them reach the solution, and the pivoting/dense/residual
+ * routines skip them explicitly.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+
+/* ------------------------------------------------------------------------ */
+/* Thomas recurrence, one strided line.  pentad_orig_cpu.cpp:73-122.         */
+/* ------------------------------------------------------------------------ */
+#define PO_THOMAS_BODY(REAL)                                                   \
+  REAL *du2 = (REAL *)malloc(sizeof(REAL) * n);                                \
+  REAL *dw2 = (REAL *)malloc(sizeof(REAL) * n);                                \
+  REAL *x2 = (REAL *)malloc(sizeof(REAL) * n);                                 \
+  /* row 0: normalise */                                                       \
+  du2[0] = du[0] / d[0];                                                       \
+  dw2[0] = dw[0] / d[0];                                                       \
+  x2[0] = x[0] / d[0];                                                         \
+  /* row 1: eliminate dl with row 0, normalise */                              \
+  REAL ddl = dl[stride];                                                       \
+  REAL dd = d[stride] - ddl * du2[0];                                          \
+  du2[1] = (du[stride] - ddl * dw2[0]) / dd;                                   \
+  dw2[1] = dw[stride] / dd;                                                    \
+  x2[1] = (x[stride] - ddl * x2[0]) / dd;                                      \
+  for (size_t i = 2; i < n; ++i) {                                             \
+    const ptrdiff_t o = (ptrdiff_t)i * stride;                                 \
+    REAL dds = ds[o];                                                          \
+    ddl = dl[o] - dds * du2[i - 2];                                            \
+    dd = d[o] - dds * dw2[i - 2] - ddl * du2[i - 1];                           \
+    du2[i] = (du[o] - ddl * dw2[i - 1]) / dd;                                  \
+    dw2[i] = dw[o] / dd;                                                       \
+    x2[i] = (x[o] - dds * x2[i - 2] - ddl * x2[i - 1]) / dd;                   \
+  }                                                                            \
+  /* backward substitution */                                                  \
+  x[(ptrdiff_t)(n - 1) * stride] = x2[n - 1];                                  \
+  x[(ptrdiff_t)(n - 2) * stride] =                                             \
+      x2[n - 2] - du2[n - 2] * x[(ptrdiff_t)(n - 1) * stride];                 \
+  for (ptrdiff_t i = (ptrdiff_t)n - 3; i >= 0; --i) {                          \
+    x[i * stride] = x2[i] - dw2[i] * x[(i + 2) * stride] -                     \
+                    du2[i] * x[(i + 1) * stride];                              \
+  }                                                                            \
+  free(du2);                                                                   \
+  free(dw2);                                                                   \
+  free(x2);
+
+void po_thomas_d(const double *ds, const double *dl, const double *d,
+                 const double *du, const double *dw, double *x, size_t n,
+                 ptrdiff_t stride) {
+  PO_THOMAS_BODY(double)
+}
+
+void po_thomas_s(const float *ds, const float *dl, const float *d,
+                 const float *du, const float *dw, float *x, size_t n,
+                 ptrdiff_t stride) {
+  PO_THOMAS_BODY(float)
+}
+
+/* Long-double line solve: reads double inputs, writes double output, carries
+ * the recurrence in long double (80-bit on x86-64). */
+void po_thomas_ld(const double *ds, const double *dl, const double *d,
+                  const double *du, const double *dw, const double *b,
+                  double *out, size_t n, ptrdiff_t stride) {
+  typedef long double REAL;
+  REAL *du2 = (REAL *)malloc(sizeof(REAL) * n);
+  REAL *dw2 = (REAL *)malloc(sizeof(REAL) * n);
+  REAL *x2 = (REAL *)malloc(sizeof(REAL) * n);
+  REAL *xs = (REAL *)malloc(sizeof(REAL) * n);
+  du2[0] = (REAL)du[0] / d[0];
+  dw2[0] = (REAL)dw[0] / d[0];
+  x2[0] = (REAL)b[0] / d[0];
+  REAL ddl = dl[stride];
+  REAL dd = d[stride] - ddl * du2[0];
+  du2[1] = (du[stride] - ddl * dw2[0]) / dd;
+  dw2[1] = dw[stride] / dd;
+  x2[1] = (b[stride] - ddl * x2[0]) / dd;
+  for (size_t i = 2; i < n; ++i) {
+    const ptrdiff_t o = (ptrdiff_t)i * stride;
+    REAL dds = ds[o];
+    ddl = dl[o] - dds * du2[i - 2];
+    dd = d[o] - dds * dw2[i - 2] - ddl * du2[i - 1];
+    du2[i] = (du[o] - ddl * dw2[i - 1]) / dd;
+    dw2[i] = dw[o] / dd;
+    x2[i] = (b[o] - dds * x2[i - 2] - ddl * x2[i - 1]) / dd;
+  }
+  xs[n - 1] = x2[n - 1];
+  xs[n - 2] = x2[n - 2] - du2[n - 2] * xs[n - 1];
+  for (ptrdiff_t i = (ptrdiff_t)n - 3; i >= 0; --i)
+    xs[i] = x2[i] - dw2[i] * xs[i + 2] - du2[i] * xs[i + 1];
+  for (size_t i = 0; i < n; ++i) out[(ptrdiff_t)i * stride] = (double)xs[i];
+  free(du2);
+  free(dw2);
+  free(x2);
+  free(xs);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Line decomposition.  pentad_orig_cpu.cpp:158-212.                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  size_t n_in;   /* lines whose first element is contiguous (stride dim)  */
+  size_t n_out;  /* outer repetitions                                     */
+  size_t len;    /* system length                                         */
+  ptrdiff_t stride;     /* distance between consecutive rows of a line    */
+  ptrdiff_t line_step;  /* distance between consecutive lines, inner      */
+  ptrdiff_t block_step; /* distance between outer repetitions             */
+} po_lines;
+
+static int po_decompose(const int *dims, int ndim, int solvedim,
+                        po_lines *L) {
+  if (ndim < 1 || solvedim < 0 || solvedim >= ndim) return -1;
+  size_t n_in = 1, n_out = 1;
+  for (int i = 0; i < solvedim; ++i) n_in *= (size_t)dims[i];
+  for (int i = solvedim + 1; i < ndim; ++i) n_out *= (size_t)dims[i];
+  L->len = (size_t)dims[solvedim];
+  if (solvedim == 0) {
+    /* contiguous lines: line j starts at j*len */
+    L->n_in = n_out; /* all lines are "inner" */
+    L->n_out = 1;
+    L->stride = 1;
+    L->line_step = (ptrdiff_t)L->len;
+    L->block_step = 0;
+  } else {
+    L->n_in = n_in;
+    L->n_out = n_out;
+    L->stride = (ptrdiff_t)n_in;
+    L->line_step = 1;
+    L->block_step = (ptrdiff_t)(n_in * L->len);
+  }
+  return 0;
+}
+
+#define PO_BATCH_BODY(SOLVE)                                                   \
+  po_lines L;                                                                  \
+  if (po_decompose(dims, ndim, solvedim, &L) != 0) return -1;                  \
+  if (L.len < 3) return -2;                                                    \
+  const long long nl = (long long)(L.n_in * L.n_out);                          \
+  _Pragma("omp parallel for schedule(static)")                                 \
+  for (long long q = 0; q < nl; ++q) {                                         \
+    const size_t o = (size_t)q / L.n_in, j = (size_t)q % L.n_in;               \
+    const ptrdiff_t base = (ptrdiff_t)o * L.block_step +                       \
+                           (ptrdiff_t)j * L.line_step;                         \
+    SOLVE(ds + base, dl + base, d + base, du + base, dw + base, x + base,      \
+          L.len, L.stride);                                                    \
+  }                                                                            \
+  return 0;
+
+int po_gpsv_batch_d(const double *ds, const double *dl, const double *d,
+                    const double *du, const double *dw, double *x,
+                    const int *dims, int ndim, int solvedim) {
+  PO_BATCH_BODY(po_thomas_d)
+}
+
+int po_gpsv_batch_s(const float *ds, const float *dl, const float *d,
+                    const float *du, const float *dw, float *x,
+                    const int *dims, int ndim, int solvedim) {
+  PO_BATCH_BODY(po_thomas_s)
+}
+
+/* long-double batch: b (input) and out (output) are separate arrays */
+int po_gpsv_batch_ld(const double *ds, const double *dl, const double *d,
+                     const double *du, const double *dw, const double *b,
+                     double *out, const int *dims, int ndim, int solvedim) {
+  po_lines L;
+  if (po_decompose(dims, ndim, solvedim, &L) != 0) return -1;
+  if (L.len < 3) return -2;
+  const long long nl = (long long)(L.n_in * L.n_out);
+#pragma omp parallel for schedule(static)
+  for (long long q = 0; q < nl; ++q) {
+    const size_t o = (size_t)q / L.n_in, j = (size_t)q % L.n_in;
+    const ptrdiff_t base =
+        (ptrdiff_t)o * L.block_step + (ptrdiff_t)j * L.line_step;
+    po_thomas_ld(ds + base, dl + base, d + base, du + base, dw + base,
+                 b + base, out + base, L.len, L.stride);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Banded Gaussian elimination with partial pivoting, one strided line.      */
+/* Row i of the matrix holds columns i-2..i+2; after row interchanges a row   */
+/* can reach column i+4 (kl+ku fill), so each working row keeps 7 entries    */
+/* for columns i-2 .. i+4.                                                    */
+/* ------------------------------------------------------------------------ */
+int po_banded_lu_d(const double *ds, const double *dl, const double *d,
+                   const double *du, const double *dw, const double *b,
+                   double *out, size_t n, ptrdiff_t stride) {
+  /* W[i][k] = A(i, i-2+k), k = 0..6 */
+  double(*W)[7] = (double(*)[7])calloc(n, sizeof(double[7]));
+  double *rhs = (double *)malloc(sizeof(double) * n);
+  if (!W || !rhs) {
+    free(W);
+    free(rhs);
+    return -3;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const ptrdiff_t o = (ptrdiff_t)i * stride;
+    W[i][0] = (i >= 2) ? ds[o] : 0.0;
+    W[i][1] = (i >= 1) ? dl[o] : 0.0;
+    W[i][2] = d[o];
+    W[i][3] = (i + 1 < n) ? du[o] : 0.0;
+    W[i][4] = (i + 2 < n) ? dw[o] : 0.0;
+    rhs[i] = b[o];
+  }
+  /* helpers to address A(r, c) for rows r >= current column */
+#define A_(r, c) W[(r)][(c) - (ptrdiff_t)(r) + 2]
+  for (size_t k = 0; k < n; ++k) {
+    /* candidate pivot rows: k, k+1, k+2 (lower bandwidth 2) */
+    size_t p = k;
+    double best = fabs(A_(k, (ptrdiff_t)k));
+    for (size_t r = k + 1; r <= k + 2 && r < n; ++r) {
+      double v = fabs(A_(r, (ptrdiff_t)k));
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (best == 0.0) {
+      free(W);
+      free(rhs);
+      return -4; /* singular */
+    }
+    if (p != k) {
+      /* swap rows k and p over columns k .. k+4 (both rows store them:
+       * row r keeps columns r-2 .. r+4 and k <= p <= k+2) */
+      for (ptrdiff_t c = (ptrdiff_t)k; c <= (ptrdiff_t)k + 4 && c < (ptrdiff_t)n;
+           ++c) {
+        double t = A_(k, c);
+        A_(k, c) = A_(p, c);
+        A_(p, c) = t;
+      }
+      double t = rhs[k];
+      rhs[k] = rhs[p];
+      rhs[p] = t;
+    }
+    const double piv = A_(k, (ptrdiff_t)k);
+    for (size_t r = k + 1; r <= k + 2 && r < n; ++r) {
+      const double m = A_(r, (ptrdiff_t)k) / piv;
+      if (m == 0.0) continue;
+      A_(r, (ptrdiff_t)k) = 0.0;
+      for (ptrdiff_t c = (ptrdiff_t)k + 1; c <= (ptrdiff_t)k + 4 && c < (ptrdiff_t)n;
+           ++c) {
+        A_(r, c) -= m * A_(k, c);
+      }
+      rhs[r] -= m * rhs[k];
+    }
+  }
+  /* back substitution: row k has entries up to column k+4 */
+  for (ptrdiff_t k = (ptrdiff_t)n - 1; k >= 0; --k) {
+    double s = rhs[k];
+    for (ptrdiff_t c = k + 1; c <= k + 4 && c < (ptrdiff_t)n; ++c)
+      s -= A_(k, c) * rhs[c];
+    rhs[k] = s / A_(k, k);
+  }
+#undef A_
+  for (size_t i = 0; i < n; ++i) out[(ptrdiff_t)i * stride] = rhs[i];
+  free(W);
+  free(rhs);
+  return 0;
+}
+
+/* Dense LU with partial pivoting on the assembled n x n matrix. */
+int po_dense_lu_d(const double *ds, const double *dl, const double *d,
+                  const double *du, const double *dw, const double *b,
+                  double *out, size_t n, ptrdiff_t stride) {
+  double *A = (double *)calloc(n * n, sizeof(double));
+  double *r = (double *)malloc(sizeof(double) * n);
+  if (!A || !r) {
+    free(A);
+    free(r);
+    return -3;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const ptrdiff_t o = (ptrdiff_t)i * stride;
+    if (i >= 2) A[i * n + i - 2] = ds[o];
+    if (i >= 1) A[i * n + i - 1] = dl[o];
+    A[i * n + i] = d[o];
+    if (i + 1 < n) A[i * n + i + 1] = du[o];
+    if (i + 2 < n) A[i * n + i + 2] = dw[o];
+    r[i] = b[o];
+  }
+  for (size_t k = 0; k < n; ++k) {
+    size_t p = k;
+    for (size_t i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + k]) > fabs(A[p * n + k])) p = i;
+    if (A[p * n + k] == 0.0) {
+      free(A);
+      free(r);
+      return -4;
+    }
+    if (p != k) {
+      for (size_t c = 0; c < n; ++c) {
+        double t = A[k * n + c];
+        A[k * n + c] = A[p * n + c];
+        A[p * n + c] = t;
+      }
+      double t = r[k];
+      r[k] = r[p];
+      r[p] = t;
+    }
+    for (size_t i = k + 1; i < n; ++i) {
+      const double m = A[i * n + k] / A[k * n + k];
+      if (m == 0.0) continue;
+      for (size_t c = k; c < n; ++c) A[i * n + c] -= m * A[k * n + c];
+      r[i] -= m * r[k];
+    }
+  }
+  for (ptrdiff_t k = (ptrdiff_t)n - 1; k >= 0; --k) {
+    double s = r[k];
+    for (size_t c = (size_t)k + 1; c < n; ++c) s -= A[(size_t)k * n + c] * r[c];
+    r[k] = s / A[(size_t)k * n + (size_t)k];
+  }
+  for (size_t i = 0; i < n; ++i) out[(ptrdiff_t)i * stride] = r[i];
+  free(A);
+  free(r);
+  return 0;
+}
+
+/* Batched pivoting solve over the same line decomposition (method 0 = banded
+ * LU with partial pivoting, 1 = dense LU with partial pivoting). */
+int po_gpsv_batch_pivot_d(const double *ds, const double *dl, const double *d,
+                          const double *du, const double *dw, const double *b,
+                          double *out, const int *dims, int ndim, int solvedim,
+                          int method) {
+  po_lines L;
+  if (po_decompose(dims, ndim, solvedim, &L) != 0) return -1;
+  const long long nl = (long long)(L.n_in * L.n_out);
+  int status = 0;
+#pragma omp parallel for schedule(static) reduction(min : status)
+  for (long long q = 0; q < nl; ++q) {
+    const size_t o = (size_t)q / L.n_in, j = (size_t)q % L.n_in;
+    const ptrdiff_t base =
+        (ptrdiff_t)o * L.block_step + (ptrdiff_t)j * L.line_step;
+    int s = (method == 0)
+                ? po_banded_lu_d(ds + base, dl + base, d + base, du + base,
+                                 dw + base, b + base, out + base, L.len,
+                                 L.stride)
+                : po_dense_lu_d(ds + base, dl + base, d + base, du + base,
+                                dw + base, b + base, out + base, L.len,
+                                L.stride);
+    if (s < status) status = s;
+  }
+  return status;
+}
+
+/* Max-norm backward residual max|A x - b| over all lines, computed in long
+ * double, skipping out-of-range couplings. */
+double po_residual_d(const double *ds, const double *dl, const double *d,
+                     const double *du, const double *dw, const double *x,
+                     const double *b, const int *dims, int ndim,
+                     int solvedim) {
+  po_lines L;
+  if (po_decompose(dims, ndim, solvedim, &L) != 0) return -1.0;
+  const long long nl = (long long)(L.n_in * L.n_out);
+  double worst = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : worst)
+  for (long long q = 0; q < nl; ++q) {
+    const size_t o = (size_t)q / L.n_in, j = (size_t)q % L.n_in;
+    const ptrdiff_t base =
+        (ptrdiff_t)o * L.block_step + (ptrdiff_t)j * L.line_step;
+    const ptrdiff_t st = L.stride;
+    const size_t n = L.len;
+    for (size_t i = 0; i < n; ++i) {
+      const ptrdiff_t e = base + (ptrdiff_t)i * st;
+      long double acc = (long double)d[e] * x[e];
+      if (i >= 2) acc += (long double)ds[e] * x[e - 2 * st];
+      if (i >= 1) acc += (long double)dl[e] * x[e - st];
+      if (i + 1 < n) acc += (long double)du[e] * x[e + st];
+      if (i + 2 < n) acc += (long double)dw[e] * x[e + 2 * st];
+      double r = (double)fabsl(acc - (long double)b[e]);
+      if (r > worst) worst = r;
+    }
+  }
+  return worst;
+}
+
+/* Variable-length batch: system k occupies [offsets[k], offsets[k+1]) of each
+ * array, contiguous.  Each system is solved on its own by the Thomas line
+ * solve (the "solve each system separately" comparison). */
+int po_varlen_d(const double *ds, const double *dl, const double *d,
+                const double *du, const double *dw, double *x,
+                const int64_t *offsets, int64_t nsys) {
+  int status = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(min : status)
+  for (int64_t k = 0; k < nsys; ++k) {
+    const int64_t b = offsets[k], n = offsets[k + 1] - offsets[k];
+    if (n < 3) {
+      status = -2;
+      continue;
+    }
+    po_thomas_d(ds + b, dl + b, d + b, du + b, dw + b, x + b, (size_t)n, 1);
+  }
+  return status;
+}
+
+int po_varlen_ld(const double *ds, const double *dl, const double *d,
+                 const double *du, const double *dw, const double *bvec,
+                 double *out, const int64_t *offsets, int64_t nsys) {
+  int status = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(min : status)
+  for (int64_t k = 0; k < nsys; ++k) {
+    const int64_t b = offsets[k], n = offsets[k + 1] - offsets[k];
+    if (n < 3) {
+      status = -2;
+      continue;
+    }
+    po_thomas_ld(ds + b, dl + b, d + b, du + b, dw + b, bvec + b, out + b,
+                 (size_t)n, 1);
+  }
+  return status;
+}

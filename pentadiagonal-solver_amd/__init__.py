@@ -1,0 +1,211 @@
+"""Python mirror of the pentadsolver C ABI (libpentadsolver.so, gfx950).
+
+This module is the host-side counterpart of the reference's public interface
+(include/pentadsolver.hpp:36-131 of SidhantKrAgrawal/pentadiagonal-solver):
+``gpsv_batch`` solves every pentadiagonal system along ``solvedim`` in place,
+``gpsv_batch_buffer_size_ext`` reports the scratch bytes, ``Handle`` wraps
+``pentadsolver_create`` / ``pentadsolver_destroy``, and ``kernel_that_ran``
+returns the executed-kernel record.  Device memory comes from torch tensors
+(the arrays' ``data_ptr``); torch is only plumbing, every solve runs in the
+HIP library.
+
+The library must be built (``make -C pentadiagonal-solver_amd``); importing
+this module does not build it and ``load_library`` raises when it is missing,
+so a missing extension can never be mistaken for a result.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libpentadsolver.so")
+
+_lib = None
+
+# C entry points declared in include/pentadsolver.h: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_SZ = ctypes.c_size_t
+_IP = ctypes.POINTER(ctypes.c_int)
+
+_SIGNATURES = {
+    "pentadsolver_create": (None, [ctypes.POINTER(_P), _P, _I, _IP]),
+    "pentadsolver_destroy": (None, [ctypes.POINTER(_P)]),
+    "pentadsolver_D_gpsv_batch_buffer_size_ext": (_SZ, [_P] * 7 + [_IP, _I, _I]),
+    "pentadsolver_S_gpsv_batch_buffer_size_ext": (_SZ, [_P] * 7 + [_IP, _I, _I]),
+    "pentadsolver_D_gpsv_batch": (None, [_P] * 7 + [_IP, _I, _I, _P]),
+    "pentadsolver_S_gpsv_batch": (None, [_P] * 7 + [_IP, _I, _I, _P]),
+    "pentadsolver_kernel_that_ran": (ctypes.c_char_p, [_I]),
+    "pentadsolver_get_last_status": (_I, [_P]),
+    "pentadsolver_set_stream": (_I, [_P, _P]),
+    "pentadsolver_set_algo": (_I, [_P, _I, ctypes.c_char_p]),
+    "pentadsolver_freeze_env": (_I, [_P, _I]),
+    "pentadsolver_kernel_config": (ctypes.c_char_p, [_P, _I]),
+    "pentadsolver_set_kernel_params": (_I, [_P, _I, _I, _I]),
+}
+
+STATUS_NAMES = {
+    0: "SUCCESS", 1: "NOT_INITIALIZED", 2: "ALLOC_FAILED", 3: "INVALID_VALUE",
+    4: "ARCH_MISMATCH", 5: "MAPPING_ERROR", 6: "EXECUTION_FAILED",
+    7: "INTERNAL_ERROR", 8: "MATRIX_TYPE_NOT_SUPPORTED", 9: "ZERO_PIVOT",
+}
+
+
+class PentadError(RuntimeError):
+    """A library call reported a non-success pentadStatus_t."""
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libpentadsolver.so and bind the declared signatures.
+
+    Raises FileNotFoundError when the library has not been built.
+    """
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `make -C {PKG_DIR}` "
+            "(no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def exported_symbols() -> Sequence[str]:
+    return tuple(_SIGNATURES)
+
+
+def _dims_arg(dims: Sequence[int]):
+    arr = (ctypes.c_int * len(dims))(*[int(v) for v in dims])
+    return arr
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def _prefix(dtype) -> str:
+    import torch
+    if dtype == torch.float64:
+        return "D"
+    if dtype == torch.float32:
+        return "S"
+    raise TypeError(f"unsupported dtype {dtype}: float32 or float64")
+
+
+class Handle:
+    """pentadsolver_create / pentadsolver_destroy as a context manager."""
+
+    def __init__(self, stream=None):
+        lib = load_library()
+        self._h = _P()
+        lib.pentadsolver_create(ctypes.byref(self._h), None, 0, None)
+        if stream is not None:
+            self.set_stream(stream)
+
+    @property
+    def value(self):
+        return self._h
+
+    def set_stream(self, stream) -> None:
+        """stream: a torch.cuda.Stream (or raw hipStream_t integer)."""
+        raw = getattr(stream, "cuda_stream", stream)
+        _check(load_library().pentadsolver_set_stream(self._h, _P(int(raw))))
+
+    def set_algo(self, direction: int, algo: Optional[str]) -> None:
+        enc = None if algo is None else algo.encode()
+        _check(load_library().pentadsolver_set_algo(self._h, direction, enc))
+
+    def freeze_env(self, frozen: bool = True) -> None:
+        _check(load_library().pentadsolver_freeze_env(self._h, int(frozen)))
+
+    def set_kernel_params(self, direction: int, rows: int, width: int) -> None:
+        _check(load_library().pentadsolver_set_kernel_params(
+            self._h, direction, rows, width))
+
+    def kernel_config(self, direction: int) -> str:
+        return load_library().pentadsolver_kernel_config(self._h, direction).decode()
+
+    def last_status(self) -> int:
+        return load_library().pentadsolver_get_last_status(self._h)
+
+    def close(self) -> None:
+        if self._h:
+            load_library().pentadsolver_destroy(ctypes.byref(self._h))
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _hval(handle) -> Optional[int]:
+    if handle is None:
+        return None
+    return handle.value if isinstance(handle, Handle) else handle
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise PentadError(f"pentadsolver status {STATUS_NAMES.get(status, status)}")
+
+
+def gpsv_batch_buffer_size_ext(ds, dl, d, du, dw, x, dims: Sequence[int],
+                               solvedim: int, handle=None) -> int:
+    """Bytes of scratch the solve needs (reference include/pentadsolver.hpp:64-73)."""
+    lib = load_library()
+    fn = getattr(lib, f"pentadsolver_{_prefix(x.dtype)}_gpsv_batch_buffer_size_ext")
+    return int(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw),
+                  _ptr(x), _dims_arg(dims), len(dims), int(solvedim)))
+
+
+def gpsv_batch(ds, dl, d, du, dw, x, dims: Sequence[int], solvedim: int,
+               buffer=None, handle=None, check: bool = True) -> None:
+    """Solve in place along ``solvedim`` (reference include/pentadsolver.hpp:107-118).
+
+    All arrays are device tensors of one dtype with prod(dims) elements;
+    dims[0] is the fastest dimension.  ``buffer`` is a device tensor of at
+    least ``gpsv_batch_buffer_size_ext`` bytes, or None.  Asynchronous on the
+    handle's stream.  Raises PentadError when the library reports an error.
+    """
+    lib = load_library()
+    for t in (ds, dl, d, du, dw):
+        if t.dtype != x.dtype:
+            raise TypeError("all arrays must share one dtype")
+    fn = getattr(lib, f"pentadsolver_{_prefix(x.dtype)}_gpsv_batch")
+    fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+       _dims_arg(dims), len(dims), int(solvedim), _ptr(buffer))
+    if check:
+        _check(lib.pentadsolver_get_last_status(_hval(handle)))
+
+
+def solve(ds, dl, d, du, dw, x, dims: Sequence[int], solvedim: int, handle=None):
+    """gpsv_batch with the scratch buffer allocated as needed (convenience)."""
+    import torch
+    nbytes = gpsv_batch_buffer_size_ext(ds, dl, d, du, dw, x, dims, solvedim, handle)
+    buf = None
+    if nbytes:
+        buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, buf, handle)
+    return buf
+
+
+def kernel_that_ran(direction: int) -> str:
+    """Reference pentadsolver_kernel_that_ran (pentad_cuda.cu:1159-1161)."""
+    return load_library().pentadsolver_kernel_that_ran(direction).decode()

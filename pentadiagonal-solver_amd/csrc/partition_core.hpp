@@ -1,0 +1,245 @@
+// partition_core.hpp -- register-resident partitioned pentadiagonal solve.
+//
+// One system of length len is cut into P chunks of M consecutive rows; one
+// lane owns one chunk and keeps every row of it in registers.  The last two
+// rows of each chunk are its "separator" S_p; the first M-2 rows are its
+// interior.  Because the matrix has bandwidth 2, removing the separators
+// decouples the interiors, so:
+//
+//  1. each lane eliminates its interior (banded LU without pivoting, the
+//     recurrence of src/cpu/singlenode/pentad_orig_cpu.cpp:94-104) carrying
+//     three right-hand sides: the RHS f and the couplings to the previous
+//     separator S_{p-1} = (x[r0-2], x[r0-1]);
+//  2. a backward sweep expresses the interior's first two and last two rows
+//     as affine functions of (S_{p-1}, S_p);
+//  3. substituting them into the two separator equations gives a block
+//     tridiagonal system in the S_p with 2x2 blocks, one block row per lane,
+//     solved by block parallel cyclic reduction in ceil(log2 P) rounds;
+//  4. with S_{p-1} and S_p known, a final backward sweep over the interior
+//     gives the solution.
+//
+// This is the capability of the reference's hybrid Thomas-PCR
+// (src/cuda/singlenode/pentad_cuda.cu:236-611), formulated differently:
+// separators and 2x2 blocks instead of per-chunk 4x4 blocks, so the reduced
+// system is half the size and needs no 4x4 Gauss-Jordan.  Out-of-range
+// couplings (ds, dl of the first rows, du, dw of the last rows) are never
+// read into the arithmetic, matching the reference's don't-care convention.
+//
+// The exchange between the lanes of one system is abstracted by Ex:
+//   Ex::template swap<K>(const T *send, T *from_prev, T *from_next, int h)
+// delivers the K values of lane p-h and of lane p+h (zeros when out of
+// range).  Wavefront shuffles implement it for x, LDS for y/z.
+#pragma once
+
+#include "pentad_device.hpp"
+
+namespace pentad {
+
+// 2x2 helpers (row-major a00 a01 a10 a11)
+template <typename T> struct M2 {
+  T a00, a01, a10, a11;
+};
+template <typename T> __device__ __forceinline__ M2<T> mul(const M2<T> &x, const M2<T> &y) {
+  return {x.a00 * y.a00 + x.a01 * y.a10, x.a00 * y.a01 + x.a01 * y.a11,
+          x.a10 * y.a00 + x.a11 * y.a10, x.a10 * y.a01 + x.a11 * y.a11};
+}
+template <typename T> __device__ __forceinline__ M2<T> inv(const M2<T> &b) {
+  const T r = recip(b.a00 * b.a11 - b.a01 * b.a10);
+  return {b.a11 * r, -b.a01 * r, -b.a10 * r, b.a00 * r};
+}
+
+// Solve the chunk owned by this lane.  On entry s,l,d,u,w,f hold rows
+// r0..r0+M-1 of the five diagonals and the RHS; on return f holds the
+// solution of those rows.  p is the chunk index, P the chunk count.
+template <typename T, int M, typename Ex>
+__device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
+                                           const T (&d)[M], const T (&u)[M],
+                                           const T (&w)[M], T (&f)[M], int p,
+                                           int P, Ex &ex) {
+  static_assert(M >= 4, "a chunk needs at least two interior rows");
+  constexpr int m = M - 2; // interior rows 0..m-1, separator rows m, m+1
+  const bool first = (p == 0);
+  const bool last = (p == P - 1);
+
+  // ---- 1. forward elimination of the interior, 3 right-hand sides --------
+  // After it, x_i + du2_i x_{i+1} + dw2_i x_{i+2} = zf_i + za_i s0 + zb_i s1
+  // where (s0, s1) = S_{p-1}.
+  T du2[m], dw2[m], zf[m], za[m], zb[m];
+  {
+    const T iv = recip(d[0]);
+    du2[0] = u[0] * iv;
+    dw2[0] = w[0] * iv;
+    zf[0] = f[0] * iv;
+    za[0] = first ? T(0) : -s[0] * iv;
+    zb[0] = first ? T(0) : -l[0] * iv;
+  }
+  {
+    const T ddl = l[1];
+    const T iv = recip(d[1] - ddl * du2[0]);
+    du2[1] = (u[1] - ddl * dw2[0]) * iv;
+    dw2[1] = w[1] * iv;
+    zf[1] = (f[1] - ddl * zf[0]) * iv;
+    za[1] = (-ddl * za[0]) * iv;
+    zb[1] = ((first ? T(0) : -s[1]) - ddl * zb[0]) * iv;
+  }
+#pragma unroll
+  for (int i = 2; i < m; ++i) {
+    const T ddl = l[i] - s[i] * du2[i - 2];
+    const T iv = recip(d[i] - s[i] * dw2[i - 2] - ddl * du2[i - 1]);
+    du2[i] = (u[i] - ddl * dw2[i - 1]) * iv;
+    dw2[i] = w[i] * iv;
+    zf[i] = (f[i] - s[i] * zf[i - 2] - ddl * zf[i - 1]) * iv;
+    za[i] = (-s[i] * za[i - 2] - ddl * za[i - 1]) * iv;
+    zb[i] = (-s[i] * zb[i - 2] - ddl * zb[i - 1]) * iv;
+  }
+
+  // ---- 2. backward sweep with 5 components [f, s0, s1, t0, t1] -----------
+  // (t0, t1) = S_p = (x_m, x_{m+1}).  Keep rows m-2, m-1 (XL2, XL1) and
+  // rows 0, 1 (XF0, XF1).
+  T XL1[5], XL2[5], XF0[5], XF1[5];
+  {
+    const int i = m - 1;
+    XL1[0] = zf[i];
+    XL1[1] = za[i];
+    XL1[2] = zb[i];
+    XL1[3] = -du2[i];
+    XL1[4] = -dw2[i];
+  }
+  {
+    const int i = m - 2;
+    XL2[0] = zf[i] - du2[i] * XL1[0];
+    XL2[1] = za[i] - du2[i] * XL1[1];
+    XL2[2] = zb[i] - du2[i] * XL1[2];
+    XL2[3] = -du2[i] * XL1[3] - dw2[i];
+    XL2[4] = -du2[i] * XL1[4];
+  }
+  {
+    T xa[5], xb[5]; // rows i+1, i+2
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      xa[c] = XL2[c];
+      xb[c] = XL1[c];
+    }
+#pragma unroll
+    for (int i = m - 3; i >= 0; --i) {
+      const T z3[3] = {zf[i], za[i], zb[i]};
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        const T zc = (c < 3) ? z3[c] : T(0);
+        const T xi = zc - du2[i] * xa[c] - dw2[i] * xb[c];
+        xb[c] = xa[c];
+        xa[c] = xi;
+      }
+    }
+    // now xa = row 0, xb = row 1 (for m == 2 these are XL2, XL1)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      XF0[c] = xa[c];
+      XF1[c] = xb[c];
+    }
+  }
+
+  // ---- 3. separator equations -> one 2x2 block row per lane --------------
+  T Y0[5], Y1[5]; // next chunk's interior rows 0, 1 (zeros for the last)
+  {
+    T send[10], rp[10], rn[10];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      send[c] = XF0[c];
+      send[5 + c] = XF1[c];
+    }
+    ex.template swap<10>(send, rp, rn, 1);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      Y0[c] = rn[c];
+      Y1[c] = rn[5 + c];
+    }
+  }
+  // separator rows m (eq a) and m+1 (eq b); couplings past the end of the
+  // system are out of range for the last chunk.
+  const T sa = s[m], la = l[m], da = d[m], ua = u[m];
+  const T wa = last ? T(0) : w[m];
+  const T sb = s[m + 1], lb = l[m + 1], db = d[m + 1];
+  const T ub = last ? T(0) : u[m + 1];
+  const T wb = last ? T(0) : w[m + 1];
+
+  M2<T> A, B, C;
+  T r0, r1;
+  // eq a: sa x_{m-2} + la x_{m-1} + da t0 + ua t1 + wa y0 = f_m
+  A.a00 = sa * XL2[1] + la * XL1[1];
+  A.a01 = sa * XL2[2] + la * XL1[2];
+  B.a00 = sa * XL2[3] + la * XL1[3] + da + wa * Y0[1];
+  B.a01 = sa * XL2[4] + la * XL1[4] + ua + wa * Y0[2];
+  C.a00 = wa * Y0[3];
+  C.a01 = wa * Y0[4];
+  r0 = f[m] - sa * XL2[0] - la * XL1[0] - wa * Y0[0];
+  // eq b: sb x_{m-1} + lb t0 + db t1 + ub y0 + wb y1 = f_{m+1}
+  A.a10 = sb * XL1[1];
+  A.a11 = sb * XL1[2];
+  B.a10 = sb * XL1[3] + lb + ub * Y0[1] + wb * Y1[1];
+  B.a11 = sb * XL1[4] + db + ub * Y0[2] + wb * Y1[2];
+  C.a10 = ub * Y0[3] + wb * Y1[3];
+  C.a11 = ub * Y0[4] + wb * Y1[4];
+  r1 = f[m + 1] - sb * XL1[0] - ub * Y0[0] - wb * Y1[0];
+
+  // normalise: S_p + A S_{p-1} + C S_{p+1} = r
+  {
+    const M2<T> Bi = inv(B);
+    A = mul(Bi, A);
+    C = mul(Bi, C);
+    const T t0 = Bi.a00 * r0 + Bi.a01 * r1;
+    const T t1 = Bi.a10 * r0 + Bi.a11 * r1;
+    r0 = t0;
+    r1 = t1;
+  }
+
+  // ---- block parallel cyclic reduction -----------------------------------
+  for (int h = 1; h < P; h <<= 1) {
+    T send[10], rp[10], rn[10];
+    send[0] = A.a00; send[1] = A.a01; send[2] = A.a10; send[3] = A.a11;
+    send[4] = C.a00; send[5] = C.a01; send[6] = C.a10; send[7] = C.a11;
+    send[8] = r0;    send[9] = r1;
+    ex.template swap<10>(send, rp, rn, h);
+    const M2<T> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
+    const M2<T> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
+    // B' = I - A Cp - C An ; A' = -A Ap ; C' = -C Cn ; r' = r - A rp - C rn
+    const M2<T> ACp = mul(A, Cp), CAn = mul(C, An);
+    M2<T> Bn{T(1) - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
+             -ACp.a10 - CAn.a10, T(1) - ACp.a11 - CAn.a11};
+    const T q0 = r0 - (A.a00 * rp[8] + A.a01 * rp[9]) -
+                 (C.a00 * rn[8] + C.a01 * rn[9]);
+    const T q1 = r1 - (A.a10 * rp[8] + A.a11 * rp[9]) -
+                 (C.a10 * rn[8] + C.a11 * rn[9]);
+    M2<T> An2 = mul(A, Ap), Cn2 = mul(C, Cn);
+    const M2<T> Bi = inv(Bn);
+    const M2<T> nA{-An2.a00, -An2.a01, -An2.a10, -An2.a11};
+    const M2<T> nC{-Cn2.a00, -Cn2.a01, -Cn2.a10, -Cn2.a11};
+    A = mul(Bi, nA);
+    C = mul(Bi, nC);
+    r0 = Bi.a00 * q0 + Bi.a01 * q1;
+    r1 = Bi.a10 * q0 + Bi.a11 * q1;
+  }
+  const T t0 = r0, t1 = r1; // S_p
+
+  // ---- 4. previous separator, final backward sweep -----------------------
+  T s0, s1;
+  {
+    T send[2] = {t0, t1}, rp[2], rn[2];
+    ex.template swap<2>(send, rp, rn, 1);
+    s0 = rp[0];
+    s1 = rp[1];
+  }
+  T xa = t0, xb = t1; // x_{i+1}, x_{i+2}
+#pragma unroll
+  for (int i = m - 1; i >= 0; --i) {
+    const T hi = zf[i] + za[i] * s0 + zb[i] * s1;
+    const T xi = hi - du2[i] * xa - dw2[i] * xb;
+    f[i] = xi;
+    xb = xa;
+    xa = xi;
+  }
+  f[m] = t0;
+  f[m + 1] = t1;
+}
+
+} // namespace pentad

@@ -1,0 +1,444 @@
+// pentad_api.cpp -- C ABI, handle and dispatch of libpentadsolver.
+//
+// Replaces the host half of src/cuda/singlenode/pentad_cuda.cu (:1129-1652):
+// direction routing, line decomposition, per-direction algorithm selection
+// with the PENTA_*ALGO environment names, the executed-kernel record and the
+// buffer sizing.  Differences by design: state lives in a handle (stream,
+// overrides, tuning, record) instead of process globals, dispatch reports a
+// status, and the buffer size is the true need of the kernel that will run.
+#include "pentad_api_internal.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace pentad {
+
+int decompose(const int *dims, int ndim, int solvedim, Lines *out) {
+  if (dims == nullptr || ndim < 1 || solvedim < 0 || solvedim >= ndim)
+    return -1;
+  int64_t n_in = 1, n_out = 1;
+  for (int i = 0; i < ndim; ++i)
+    if (dims[i] < 1) return -1;
+  for (int i = 0; i < solvedim; ++i) n_in *= dims[i];
+  for (int i = solvedim + 1; i < ndim; ++i) n_out *= dims[i];
+  Lines ln;
+  ln.len = dims[solvedim];
+  ln.n_sys = n_in * n_out;
+  if (solvedim == 0) {
+    ln.dir = 0;
+    ln.stride = 1;
+    ln.n_in = 1;
+  } else {
+    ln.dir = (solvedim == ndim - 1) ? 2 : 1;
+    ln.stride = n_in;
+    ln.n_in = n_in;
+  }
+  *out = ln;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// contexts
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_ran_mu;
+const char *g_ran[3] = {"none", "none", "none"};
+Context g_default;
+} // namespace
+
+Context *ctx_of(pentadsolver_handle_t h) {
+  return h ? static_cast<Context *>(h) : &g_default;
+}
+
+void note_kernel(Context *c, int dir, const char *name, const std::string &cfg) {
+  if (dir < 0 || dir > 2) return;
+  {
+    std::lock_guard<std::mutex> lk(g_ran_mu);
+    g_ran[dir] = name;
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->ran[dir] = name;
+  c->ran_cfg[dir] = cfg;
+}
+
+const char *ran_global(int dir) {
+  std::lock_guard<std::mutex> lk(g_ran_mu);
+  return (dir >= 0 && dir < 3) ? g_ran[dir] : "none";
+}
+
+void set_status(Context *c, pentadStatus_t s) { c->last_status = s; }
+
+void destroy_context_resources(Context * /*c*/) {}
+
+// Algorithm requested for a direction: handle override, then the
+// environment with the reference's precedence (PENTA_YALGO / PENTA_ZALGO
+// over PENTA_YZALGO, src/cuda/singlenode/pentad_cuda.cu:1163-1170), then
+// "auto".
+std::string requested_algo(Context *c, int dir) {
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->algo[dir].empty()) return c->algo[dir];
+    if (c->env_frozen) return c->env_algo[dir];
+  }
+  return env_algo(dir);
+}
+
+std::string env_algo(int dir) {
+  const char *v = nullptr;
+  if (dir == 0) {
+    v = std::getenv("PENTA_XALGO");
+  } else {
+    v = std::getenv(dir == 1 ? "PENTA_YALGO" : "PENTA_ZALGO");
+    if (v == nullptr || v[0] == '\0') v = std::getenv("PENTA_YZALGO");
+  }
+  if (v == nullptr || v[0] == '\0') return "auto";
+  std::string s(v);
+  if (s == "legacy") s = "naive";
+  return s;
+}
+
+bool debug_launch() {
+  const char *v = std::getenv("PENTA_DEBUG_LAUNCH");
+  return v != nullptr && v[0] == '1';
+}
+
+// ---------------------------------------------------------------------------
+// plan: which kernel a solve will launch
+// ---------------------------------------------------------------------------
+static bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
+                const void *const ptrs[6]) {
+  Plan pl;
+  const std::string want = requested_algo(c, ln.dir);
+  const bool is64 = (elem_bytes == 8);
+  const TuneKey key{elem_bytes, ln.dir, ln.len};
+  TuneVal tuned;
+  bool have_tuned = false;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->tune.find(key);
+    if (it != c->tune.end()) {
+      tuned = it->second;
+      have_tuned = true;
+    }
+    if (c->force_rows[ln.dir] > 0) { // explicit parameters win over tuning
+      tuned.kind = ln.dir == 0 ? Plan::PartX : Plan::PartYZ;
+      tuned.rows = c->force_rows[ln.dir];
+      tuned.lanes = c->force_width[ln.dir];
+      tuned.sys_per_wg = c->force_width[ln.dir];
+      have_tuned = true;
+    }
+  }
+
+  if (want == "naive") {
+    pl.kind = Plan::Naive;
+    return pl;
+  }
+
+  if (ln.dir == 0) {
+    bool al = true;
+    for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
+    if (al && (want == "auto" || want == "thomas-pcr")) {
+      if (have_tuned && tuned.kind == Plan::PartX &&
+          part_x_supported(elem_bytes, tuned.rows, tuned.lanes) &&
+          (int64_t)tuned.rows * tuned.lanes == ln.len) {
+        pl.kind = Plan::PartX;
+        pl.rows = tuned.rows;
+        pl.lanes = tuned.lanes;
+        return pl;
+      }
+      // preferred rows per lane: 8 (FP64) / 16 (FP32), then the others
+      static const int pref64[] = {8, 6, 10, 12, 4, 16};
+      static const int pref32[] = {16, 12, 8, 4};
+      const int *pref = is64 ? pref64 : pref32;
+      const int npref = is64 ? 6 : 4;
+      for (int k = 0; k < npref; ++k) {
+        const int r = pref[k];
+        if (ln.len % r) continue;
+        const int64_t lanes = ln.len / r;
+        if (lanes > 64 || lanes < 2 || (lanes & (lanes - 1))) continue;
+        if (!part_x_supported(elem_bytes, r, (int)lanes)) continue;
+        pl.kind = Plan::PartX;
+        pl.rows = r;
+        pl.lanes = (int)lanes;
+        return pl;
+      }
+    }
+    pl.kind = Plan::Naive;
+    return pl;
+  }
+
+  // strided directions
+  if (want == "auto" || want == "thomas-pcr") {
+    if (have_tuned && tuned.kind == Plan::PartYZ &&
+        part_yz_supported(elem_bytes, tuned.rows, tuned.sys_per_wg, ln.len)) {
+      pl.kind = Plan::PartYZ;
+      pl.rows = tuned.rows;
+      pl.sys_per_wg = tuned.sys_per_wg;
+      return pl;
+    }
+    static const int pref64[] = {8, 6, 12, 4, 16};
+    static const int pref32[] = {16, 12, 8, 6, 4};
+    const int *pref = is64 ? pref64 : pref32;
+    for (int k = 0; k < 5; ++k) {
+      const int r = pref[k];
+      for (int S = 64; S >= 16; S >>= 1) {
+        if (part_yz_supported(elem_bytes, r, S, ln.len)) {
+          pl.kind = Plan::PartYZ;
+          pl.rows = r;
+          pl.sys_per_wg = S;
+          return pl;
+        }
+      }
+    }
+  }
+  pl.kind = Plan::Naive;
+  return pl;
+}
+
+size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes) {
+  if (pl.kind == Plan::Naive)
+    return (size_t)3 * (size_t)ln.n_sys * (size_t)ln.len * (size_t)elem_bytes;
+  return 0;
+}
+
+std::string plan_config(const Plan &pl) {
+  char buf[96];
+  switch (pl.kind) {
+  case Plan::PartX:
+    std::snprintf(buf, sizeof buf, "thomas-pcr rows=%d lanes=%d", pl.rows,
+                  pl.lanes);
+    return buf;
+  case Plan::PartYZ:
+    std::snprintf(buf, sizeof buf, "thomas-pcr rows=%d sys_per_wg=%d", pl.rows,
+                  pl.sys_per_wg);
+    return buf;
+  default:
+    return "naive";
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the batched solve
+// ---------------------------------------------------------------------------
+template <typename T>
+pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
+                          const T *du, const T *dw, T *x, const int *dims,
+                          int ndim, int solvedim, void *buffer) {
+  Lines ln;
+  if (decompose(dims, ndim, solvedim, &ln) != 0 || ln.len < 3) {
+    std::fprintf(stderr, "[penta] invalid shape (ndim %d, solvedim %d)\n",
+                 ndim, solvedim);
+    return PENTAD_STATUS_INVALID_VALUE;
+  }
+  if (!ds || !dl || !d || !du || !dw || !x) {
+    std::fprintf(stderr, "[penta] null array pointer\n");
+    return PENTAD_STATUS_INVALID_VALUE;
+  }
+  const void *ptrs[6] = {ds, dl, d, du, dw, x};
+  Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs);
+  const Diags<T> a{ds, dl, d, du, dw};
+  hipStream_t st = c->stream;
+  hipError_t err = hipSuccess;
+
+  if (pl.kind == Plan::PartX) {
+    err = launch_part_x<T>(a, x, ln, pl.rows, pl.lanes, st);
+  } else if (pl.kind == Plan::PartYZ) {
+    err = launch_part_yz<T>(a, x, ln, pl.rows, pl.sys_per_wg, st);
+  }
+  if (pl.kind != Plan::Naive && err != hipSuccess) {
+    // an opt-in kernel that declines falls back, as the reference does
+    // (src/cuda/singlenode/pentad_cuda.cu:17-37)
+    if (debug_launch())
+      std::fprintf(stderr, "[penta] %s declined: %s -- falling back\n",
+                   plan_config(pl).c_str(), hipGetErrorString(err));
+    pl = Plan{};
+  }
+  if (pl.kind == Plan::Naive) {
+    if (buffer == nullptr) {
+      std::fprintf(stderr,
+                   "[penta] solvedim %d needs %zu bytes of scratch "
+                   "(pentadsolver_*_gpsv_batch_buffer_size_ext); got NULL\n",
+                   solvedim, plan_scratch_bytes(pl, ln, (int)sizeof(T)));
+      return PENTAD_STATUS_INVALID_VALUE;
+    }
+    err = launch_thomas_lane<T>(a, x, ln, static_cast<T *>(buffer), st);
+  }
+  if (err != hipSuccess) {
+    std::fprintf(stderr, "[penta] solvedim %d launch FAILED: %s -- output is invalid\n",
+                 solvedim, hipGetErrorString(err));
+    return PENTAD_STATUS_EXECUTION_FAILED;
+  }
+  note_kernel(c, ln.dir, pl.kind == Plan::Naive ? "naive" : "thomas-pcr",
+              plan_config(pl));
+  return PENTAD_STATUS_SUCCESS;
+}
+
+template <typename T>
+size_t buffer_size(Context *c, const T *ds, const T *dl, const T *d,
+                   const T *du, const T *dw, const T *x, const int *dims,
+                   int ndim, int solvedim) {
+  Lines ln;
+  if (decompose(dims, ndim, solvedim, &ln) != 0) return 0;
+  const void *ptrs[6] = {ds, dl, d, du, dw, x};
+  const Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs);
+  return plan_scratch_bytes(pl, ln, (int)sizeof(T));
+}
+
+} // namespace pentad
+
+using namespace pentad;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void pentadsolver_create(pentadsolver_handle_t *handle, void * /*comm*/,
+                         int /*ndims*/, const int * /*num_procs*/) {
+  if (handle == nullptr) return;
+  Context *c = new Context();
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  *handle = c;
+}
+
+void pentadsolver_destroy(pentadsolver_handle_t *handle) {
+  if (handle == nullptr || *handle == nullptr) return;
+  Context *c = static_cast<Context *>(*handle);
+  destroy_context_resources(c);
+  delete c;
+  *handle = nullptr;
+}
+
+size_t pentadsolver_D_gpsv_batch_buffer_size_ext(
+    pentadsolver_handle_t h, const double *ds, const double *dl,
+    const double *d, const double *du, const double *dw, const double *x,
+    const int *t_dims, int t_ndim, int t_solvedim) {
+  return buffer_size<double>(ctx_of(h), ds, dl, d, du, dw, x, t_dims, t_ndim,
+                             t_solvedim);
+}
+
+size_t pentadsolver_S_gpsv_batch_buffer_size_ext(
+    pentadsolver_handle_t h, const float *ds, const float *dl, const float *d,
+    const float *du, const float *dw, const float *x, const int *t_dims,
+    int t_ndim, int t_solvedim) {
+  return buffer_size<float>(ctx_of(h), ds, dl, d, du, dw, x, t_dims, t_ndim,
+                            t_solvedim);
+}
+
+void pentadsolver_D_gpsv_batch(pentadsolver_handle_t h, const double *ds,
+                               const double *dl, const double *d,
+                               const double *du, const double *dw, double *x,
+                               const int *t_dims, int t_ndim, int t_solvedim,
+                               void *t_buffer) {
+  Context *c = ctx_of(h);
+  set_status(c, gpsv_batch<double>(c, ds, dl, d, du, dw, x, t_dims, t_ndim,
+                                   t_solvedim, t_buffer));
+}
+
+void pentadsolver_S_gpsv_batch(pentadsolver_handle_t h, const float *ds,
+                               const float *dl, const float *d,
+                               const float *du, const float *dw, float *x,
+                               const int *t_dims, int t_ndim, int t_solvedim,
+                               void *t_buffer) {
+  Context *c = ctx_of(h);
+  set_status(c, gpsv_batch<float>(c, ds, dl, d, du, dw, x, t_dims, t_ndim,
+                                  t_solvedim, t_buffer));
+}
+
+const char *pentadsolver_kernel_that_ran(int dir) { return ran_global(dir); }
+
+pentadStatus_t pentadsolver_get_last_status(pentadsolver_handle_t h) {
+  return ctx_of(h)->last_status;
+}
+
+pentadStatus_t pentadsolver_set_stream(pentadsolver_handle_t h, void *stream) {
+  Context *c = ctx_of(h);
+  c->stream = static_cast<hipStream_t>(stream);
+  return PENTAD_STATUS_SUCCESS;
+}
+
+pentadStatus_t pentadsolver_set_algo(pentadsolver_handle_t h, int dir,
+                                     const char *algo) {
+  if (dir < 0 || dir > 2) return PENTAD_STATUS_INVALID_VALUE;
+  Context *c = ctx_of(h);
+  std::string s = algo ? algo : "";
+  if (s == "legacy") s = "naive";
+  if (!s.empty() && s != "auto" && s != "naive" && s != "transpose" &&
+      s != "thomas-pcr" && s != "shared-fact")
+    return PENTAD_STATUS_INVALID_VALUE;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->algo[dir] = s;
+  return PENTAD_STATUS_SUCCESS;
+}
+
+pentadStatus_t pentadsolver_freeze_env(pentadsolver_handle_t h, int frozen) {
+  Context *c = ctx_of(h);
+  std::string e[3] = {env_algo(0), env_algo(1), env_algo(2)};
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->env_frozen = frozen != 0;
+  for (int i = 0; i < 3; ++i) c->env_algo[i] = e[i];
+  return PENTAD_STATUS_SUCCESS;
+}
+
+pentadStatus_t pentadsolver_set_kernel_params(pentadsolver_handle_t h, int dir,
+                                              int rows, int width) {
+  if (dir < 0 || dir > 2 || rows < 0 || width < 0)
+    return PENTAD_STATUS_INVALID_VALUE;
+  Context *c = ctx_of(h);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->force_rows[dir] = rows;
+  c->force_width[dir] = width;
+  return PENTAD_STATUS_SUCCESS;
+}
+
+const char *pentadsolver_kernel_config(pentadsolver_handle_t h, int dir) {
+  if (dir < 0 || dir > 2) return "none";
+  Context *c = ctx_of(h);
+  std::lock_guard<std::mutex> lk(c->mu);
+  return c->ran_cfg[dir].c_str();
+}
+
+} // extern "C"
+
+// ===========================================================================
+// C++ overloads (reference include/pentadsolver.hpp:75-84, :120-131)
+// ===========================================================================
+size_t pentadsolver_gpsv_batch_buffer_size_ext(
+    pentadsolver_handle_t h, const double *ds, const double *dl,
+    const double *d, const double *du, const double *dw, const double *x,
+    const int *t_dims, int t_ndim, int t_solvedim) {
+  return pentadsolver_D_gpsv_batch_buffer_size_ext(h, ds, dl, d, du, dw, x,
+                                                   t_dims, t_ndim, t_solvedim);
+}
+
+size_t pentadsolver_gpsv_batch_buffer_size_ext(
+    pentadsolver_handle_t h, const float *ds, const float *dl, const float *d,
+    const float *du, const float *dw, const float *x, const int *t_dims,
+    int t_ndim, int t_solvedim) {
+  return pentadsolver_S_gpsv_batch_buffer_size_ext(h, ds, dl, d, du, dw, x,
+                                                   t_dims, t_ndim, t_solvedim);
+}
+
+void pentadsolver_gpsv_batch(pentadsolver_handle_t h, const double *ds,
+                             const double *dl, const double *d,
+                             const double *du, const double *dw, double *x,
+                             const int *t_dims, int t_ndim, int t_solvedim,
+                             void *t_buffer) {
+  pentadsolver_D_gpsv_batch(h, ds, dl, d, du, dw, x, t_dims, t_ndim,
+                            t_solvedim, t_buffer);
+}
+
+void pentadsolver_gpsv_batch(pentadsolver_handle_t h, const float *ds,
+                             const float *dl, const float *d, const float *du,
+                             const float *dw, float *x, const int *t_dims,
+                             int t_ndim, int t_solvedim, void *t_buffer) {
+  pentadsolver_S_gpsv_batch(h, ds, dl, d, du, dw, x, t_dims, t_ndim,
+                            t_solvedim, t_buffer);
+}

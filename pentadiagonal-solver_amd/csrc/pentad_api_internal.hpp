@@ -1,0 +1,63 @@
+// pentad_api_internal.hpp -- handle contents and dispatch helpers shared by
+// the host sources of libpentadsolver.  Not part of the public interface.
+#pragma once
+
+#include "pentad_internal.hpp"
+#include "pentadsolver.h"
+
+#include <map>
+#include <mutex>
+#include <string>
+
+namespace pentad {
+
+// Which kernel a solve launches and with what shape parameters.
+struct Plan {
+  enum Kind { Naive = 0, PartX = 1, PartYZ = 2 } kind = Naive;
+  int rows = 0;       // rows per lane (partitioned kernels)
+  int lanes = 0;      // lanes per system (x)
+  int sys_per_wg = 0; // systems per workgroup (y/z)
+};
+
+struct TuneKey {
+  int elem_bytes;
+  int dir;
+  int64_t len;
+  bool operator<(const TuneKey &o) const {
+    if (elem_bytes != o.elem_bytes) return elem_bytes < o.elem_bytes;
+    if (dir != o.dir) return dir < o.dir;
+    return len < o.len;
+  }
+};
+using TuneVal = Plan;
+
+struct Context {
+  std::mutex mu;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  pentadStatus_t last_status = PENTAD_STATUS_SUCCESS;
+  std::string algo[3];     // per-direction override ("" = none)
+  bool env_frozen = false; // use env_algo[] instead of reading getenv
+  std::string env_algo[3] = {"auto", "auto", "auto"};
+  const char *ran[3] = {"none", "none", "none"};
+  std::string ran_cfg[3] = {"none", "none", "none"};
+  std::map<TuneKey, TuneVal> tune; // autotuner results
+  int force_rows[3] = {0, 0, 0};   // pentadsolver_set_kernel_params
+  int force_width[3] = {0, 0, 0};
+};
+
+Context *ctx_of(pentadsolver_handle_t h);
+void destroy_context_resources(Context *c);
+void note_kernel(Context *c, int dir, const char *name, const std::string &cfg);
+const char *ran_global(int dir);
+void set_status(Context *c, pentadStatus_t s);
+std::string requested_algo(Context *c, int dir);
+std::string env_algo(int dir);
+bool debug_launch();
+
+Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
+                const void *const ptrs[6]);
+size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes);
+std::string plan_config(const Plan &pl);
+
+} // namespace pentad

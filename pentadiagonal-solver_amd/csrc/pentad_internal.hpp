@@ -1,0 +1,73 @@
+// pentad_internal.hpp -- types shared by the gfx950 kernels and the host
+// dispatch of libpentadsolver.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace pentad {
+
+// ---------------------------------------------------------------------------
+// Line decomposition of a batched solve along one axis.
+//
+// Element (i0, i1, ...) lives at i0 + d0*i1 + d0*d1*i2 ...; t_dims[0] is the
+// fastest axis.  A "line" (one independent system) runs along solvedim.
+// Semantics follow src/cuda/singlenode/pentad_cuda.cu:1420-1453 (and the CPU
+// pentad_orig_cpu.cpp:158-195), with 64-bit counts and offsets throughout:
+//   solvedim 0     : n_sys = prod(dims[1:]), line j starts at j*len, stride 1
+//   solvedim sd > 0: n_in = prod(dims[:sd]), n_out = prod(dims[sd+1:]);
+//                    line (o, j) starts at o*n_in*len + j, stride n_in.
+// The outermost direction is the n_out == 1 case of the same formula.
+// ---------------------------------------------------------------------------
+struct Lines {
+  int64_t n_sys = 0;  // number of independent systems
+  int64_t len = 0;    // rows per system
+  int64_t stride = 1; // element distance between consecutive rows of a line
+  int64_t n_in = 1;   // lines that are adjacent in memory (stride-1 run)
+  int dir = 0;        // 0 = x (contiguous lines), 1 = middle, 2 = outermost
+
+  // first element of line `sys`
+  __host__ __device__ __forceinline__ int64_t base(int64_t sys) const {
+    if (dir == 0) return sys * len;
+    if (n_in == n_sys) return sys; // outermost: lines are adjacent
+    const int64_t o = sys / n_in;
+    return o * n_in * len + (sys - o * n_in);
+  }
+};
+
+// Returns 0 on success, a negative code when the shape is invalid.
+int decompose(const int *dims, int ndim, int solvedim, Lines *out);
+
+// ---------------------------------------------------------------------------
+// Kernel argument bundle: the five diagonals (read only) and the in/out RHS.
+// Naming follows the reference API (include/pentadsolver.hpp:93-104):
+// ds = 2nd lower, dl = lower, d = main, du = upper, dw = 2nd upper.
+// ---------------------------------------------------------------------------
+template <typename T> struct Diags {
+  const T *ds, *dl, *d, *du, *dw;
+};
+
+// Launchers, one per kernel family.  Each returns hipSuccess or the launch
+// error; none synchronises.  Implemented in the kernels_*.hip files.
+template <typename T>
+hipError_t launch_thomas_lane(const Diags<T> &a, T *x, const Lines &ln,
+                              T *scratch, hipStream_t st);
+
+// Partitioned solve, x direction: `lanes` lanes of one wavefront per system,
+// each lane owning `rows` consecutive rows (lanes * rows == len).
+template <typename T>
+hipError_t launch_part_x(const Diags<T> &a, T *x, const Lines &ln, int rows,
+                         int lanes, hipStream_t st);
+bool part_x_supported(int elem_bytes, int rows, int lanes);
+
+// Partitioned solve, strided (y/z) directions: a workgroup owns `sys_per_wg`
+// adjacent systems and splits each into len/rows chunks, one lane per
+// (system, chunk); the reduced system is solved through LDS.
+template <typename T>
+hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
+                          int sys_per_wg, hipStream_t st);
+bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
+
+} // namespace pentad

@@ -1,0 +1,204 @@
+"""GPU parity of the batched solve against the CPU oracle (MI355X only).
+
+Bars: FP64 within 1e-12 max-norm relative forward error of the FP64 oracle;
+FP32 within 2e-5 of the FP64 oracle solving the same float32-rounded inputs
+(about 100 u32 for these well-conditioned systems) and the reference fixture
+rule (rel 1e-5 or abs 1e-6) on the shipped fixtures.  Every call goes through
+the C ABI (libpentadsolver.so via ctypes).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pentadsolver_amd as P
+from conftest import FIXTURES, FIXTURE_NAMES
+from oracle import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-5
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def relerr(a, ref):
+    a = np.asarray(a, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+
+
+def gen(dims, seed, dtype=np.float64, ratio=(2.0, 3.0)):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    ds, dl, du, dw = (rng.uniform(-1, 1, n) for _ in range(4))
+    d = rng.uniform(*ratio, n) * (abs(ds) + abs(dl) + abs(du) + abs(dw))
+    b = rng.uniform(-1, 1, n)
+    return [a.astype(dtype) for a in (ds, dl, d, du, dw, b)]
+
+
+def gpu_solve(arrs, dims, sd, handle=None):
+    t = [dev(a) for a in arrs]
+    P.solve(*t[:5], t[5], dims, sd, handle=handle)
+    torch.cuda.synchronize()
+    return t[5].cpu().numpy()
+
+
+def oracle_solve(arrs, dims, sd):
+    a64 = [a.astype(np.float64) for a in arrs]
+    return O.solve(*a64, dims, sd)
+
+
+def check(arrs, dims, sd, handle=None):
+    x = gpu_solve(arrs, dims, sd, handle)
+    ref = oracle_solve(arrs, dims, sd)
+    tol = FP64_TOL if arrs[0].dtype == np.float64 else FP32_TOL
+    err = relerr(x, ref)
+    assert np.all(np.isfinite(x))
+    assert err < tol, (dims, sd, err)
+    return err
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", FIXTURE_NAMES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("algo", ["auto", "naive", "thomas-pcr"])
+def test_reference_fixtures_on_gpu(name, dtype, algo):
+    fx = O.read_fixture(os.path.join(FIXTURES, name), dtype)
+    with P.Handle() as h:
+        for k in range(3):
+            h.set_algo(k, algo)
+        arrs = [fx[k] for k in ("ds", "dl", "d", "du", "dw", "x")]
+        x = gpu_solve(arrs, fx["dims"], fx["solvedim"], h)
+    assert O.allclose_reference_rule(fx["u"], x, dtype), name
+
+
+SHAPES = [
+    ([256, 64, 9], 0), ([256, 64, 9], 1), ([64, 256, 9], 1), ([9, 64, 256], 2),
+    ([512, 40], 0), ([40, 512], 1), ([64, 64, 64], 0), ([64, 64, 64], 1),
+    ([64, 64, 64], 2), ([128, 24, 10], 0), ([24, 128, 10], 1), ([10, 24, 128], 2),
+    ([37, 11, 5], 0), ([11, 37, 5], 1), ([5, 11, 37], 2),  # lengths with no partition
+]
+
+
+@pytest.mark.parametrize("dims,sd", SHAPES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_auto_dispatch_matches_oracle(dims, sd, dtype):
+    print(f"case dims={dims} sd={sd} dtype={np.dtype(dtype).name}", flush=True)
+    check(gen(dims, 17 + sd, dtype), dims, sd)
+
+
+@pytest.mark.parametrize("dims,sd", [([256, 64, 9], 1), ([64, 64, 64], 2), ([200, 30], 0)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_naive_kernel_matches_oracle(dims, sd, dtype):
+    with P.Handle() as h:
+        h.set_algo(sd, "naive")
+        check(gen(dims, 5, dtype), dims, sd, h)
+        assert P.kernel_that_ran(0 if sd == 0 else (2 if sd == len(dims) - 1 else 1)) == "naive"
+
+
+# every partitioned instantiation, x: (rows, lanes) with rows*lanes = len
+X_CONFIGS = {
+    np.float64: [(4, 16), (4, 64), (6, 32), (8, 8), (8, 32), (8, 64), (10, 32), (12, 16), (16, 16), (16, 32)],
+    np.float32: [(4, 16), (4, 64), (8, 8), (8, 32), (8, 64), (12, 16), (16, 16), (16, 32)],
+}
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_partition_x_all_configs(dtype):
+    with P.Handle() as h:
+        for rows, lanes in X_CONFIGS[dtype]:
+            n = rows * lanes
+            dims = [n, 37, 3]
+            print(f"x rows={rows} lanes={lanes} len={n} {np.dtype(dtype).name}", flush=True)
+            h.set_kernel_params(0, rows, lanes)
+            check(gen(dims, rows * 100 + lanes, dtype), dims, 0, h)
+            assert h.kernel_config(0) == f"thomas-pcr rows={rows} lanes={lanes}"
+
+
+YZ_CONFIGS = [(4, 16, 256), (4, 64, 64), (6, 16, 192), (8, 16, 256), (8, 32, 128),
+              (8, 64, 64), (12, 16, 384), (16, 16, 512), (16, 32, 256), (12, 64, 96)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("sd", [1, 2])
+def test_partition_yz_all_configs(dtype, sd):
+    with P.Handle() as h:
+        for rows, spw, n in YZ_CONFIGS:
+            P_ = n // rows
+            from_ok = (spw * P_) <= (1024 if (dtype == np.float32 and rows <= 12) or rows <= 4 else 512)
+            if not from_ok:
+                continue
+            dims = [40, n, 3] if sd == 1 else [40, 3, n]
+            print(f"yz sd={sd} rows={rows} spw={spw} len={n} {np.dtype(dtype).name}", flush=True)
+            h.set_kernel_params(sd, rows, spw)
+            check(gen(dims, rows * 7 + spw, dtype), dims, sd, h)
+            assert h.kernel_config(sd) == f"thomas-pcr rows={rows} sys_per_wg={spw}"
+
+
+def test_weakly_dominant_fp64():
+    dims = [256, 32, 32]
+    arrs = gen(dims, 4, np.float64, ratio=(1.05, 1.05))
+    for sd in range(3):
+        check(arrs, dims, sd)
+
+
+def test_outofrange_couplings_ignored():
+    dims = [256, 16, 16]
+    arrs = gen(dims, 8)
+    idx = np.arange(int(np.prod(dims))).reshape(16, 16, 256)
+    for sd in range(3):
+        lines = np.moveaxis(idx, 2 - sd, -1).reshape(-1, dims[sd])
+        a2 = [a.copy() for a in arrs]
+        a2[0][lines[:, :2]] = np.nan
+        a2[1][lines[:, 0]] = np.inf
+        a2[3][lines[:, -1]] = np.nan
+        a2[4][lines[:, -2:]] = -np.inf
+        x = gpu_solve(a2, dims, sd)
+        ref = oracle_solve(arrs, dims, sd)
+        assert relerr(x, ref) < FP64_TOL, sd
+
+
+def test_adi_sweep_in_place_256():
+    """x, then y, then z in place on one field with the same coefficient arrays
+    (reference apps/adi_cuda.cu:167-176), 2 iterations, vs the oracle."""
+    dims = [256, 64, 48]
+    arrs = gen(dims, 2)
+    t = [dev(a) for a in arrs]
+    ref = arrs[5].copy()
+    for _ in range(2):
+        for sd in range(3):
+            P.solve(*t[:5], t[5], dims, sd)
+            ref = O.solve(*arrs[:5], ref, dims, sd)
+    torch.cuda.synchronize()
+    assert relerr(t[5].cpu().numpy(), ref) < FP64_TOL
+
+
+def test_buffer_contract_and_status():
+    dims = [37, 8, 8]
+    arrs = gen(dims, 1)
+    t = [dev(a) for a in arrs]
+    need = P.gpsv_batch_buffer_size_ext(*t, dims, 0)
+    assert need == 3 * 37 * 64 * 8
+    with pytest.raises(P.PentadError):
+        P.gpsv_batch(*t, dims, 0, buffer=None)
+    with pytest.raises(P.PentadError):
+        P.gpsv_batch(*t, [37, 8, 8], 3)
+
+
+def test_streams_and_handles():
+    dims = [256, 32, 16]
+    arrs = gen(dims, 3)
+    ref = oracle_solve(arrs, dims, 1)
+    s = torch.cuda.Stream()
+    with P.Handle(stream=s) as h:
+        t = [dev(a) for a in arrs]
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            P.solve(*t[:5], t[5], dims, 1, handle=h)
+        s.synchronize()
+        assert relerr(t[5].cpu().numpy(), ref) < FP64_TOL
