@@ -1,14 +1,26 @@
-"""Benchmark: three-direction ADI step (x, then y, then z, in place) on one MI355X.
+"""Benchmark: the three-direction ADI step (x, then y, then z) on one MI355X.
 
-Prints ONE JSON line.  A "step" is one x->y->z sweep over the whole grid
-(inputs resident in HBM); rows = unknowns solved, summed over directions.
-The timed region brackets exactly --steps steps with synchronisation on
-both sides; the per-direction kernel durations come from HIP events
-recorded on the stream the kernels run on (the handle's stream is torch's
-current stream).  After timing, every direction is re-solved from a fresh
-right-hand side and a sample of lines is checked against the CPU oracle.
+Prints ONE JSON line (diagnostics go to stderr).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
+Workload: a 512^3 grid, 262144 systems of length 512 per direction,
+diagonally dominant seeded coefficients (off-diagonals U(-1,1), main
+diagonal = U(2,3) * sum|off|, RHS U(-1,1)), the same five coefficient arrays
+for every direction.  A "step" is one x->y->z sweep over the whole grid with
+the field and the coefficients resident in HBM; rows = unknowns solved,
+summed over the three directions.
+
+Headline ("value"): FP32 bulk solve + FP64 iterative refinement (coefficients
+stored in FP32, field in FP64).  Beside it: plain FP64 and plain FP32 on the
+same driver, and the in-place drop-in per-direction FP64 solves.  Each mode is
+captured once as a hipGraph and replayed; the timed region brackets exactly
+--steps replays with a synchronisation on both sides.  Per-direction kernel
+durations come from HIP events recorded on the handle's stream (torch's
+current stream), and the dominant kernel gives the roofline.  After timing,
+one step is re-run from the initial field and compared on the full grid with
+the CPU oracle: FP64 against the FP64 oracle, FP32+refinement against the
+FP64 oracle of the same FP32-stored coefficients widened to FP64.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--size 512]
 """
 from __future__ import annotations
 
@@ -27,9 +39,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GRows/s + % HBM roofline, FP32+refinement vs plain FP64 at 1e-12 rel err"
 HBM_PEAK_GBS = 8000.0
+BYTES_PER_ROW = {"ir": 36, "fp64": 56, "fp32": 28}
 
 
 def import_pkg():
+    if "pentadsolver_amd" in sys.modules:
+        return sys.modules["pentadsolver_amd"]
     spec = importlib.util.spec_from_file_location(
         "pentadsolver_amd", os.path.join(PKG, "__init__.py"),
         submodule_search_locations=[PKG])
@@ -43,103 +58,102 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def sample_lines(dims, sd, count, seed):
-    """Flat indices (count, len) of `count` random lines along sd."""
-    rng = np.random.default_rng(seed)
-    strides = [1]
-    for v in dims[:-1]:
-        strides.append(strides[-1] * v)
-    out = []
-    for _ in range(count):
-        pos = [int(rng.integers(0, v)) for v in dims]
-        pos[sd] = 0
-        base = sum(p * s for p, s in zip(pos, strides))
-        out.append(base + strides[sd] * np.arange(dims[sd]))
-    return np.array(out)
-
-
-def run_mode(P, torch, dims, dtype, steps, warmup, seed):
-    """Time the ADI step and each direction; return a dict of results."""
-    from pentadsolver_amd import workloads as W
+def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine, tol):
+    """Graph-replay timing of the ADI step for one mode; returns a dict."""
+    mode = {"fp64": P.ADI_FP64, "fp32": P.ADI_FP32, "ir": P.ADI_IR}[mode_name]
     n = int(np.prod(dims))
-    arrs = W.dominant_torch(dims, seed, dtype=dtype)
-    ds, dl, d, du, dw, b = arrs
-    x = b.clone()
     stream = torch.cuda.current_stream()
-    h = P.Handle(stream=stream)
-    h.freeze_env(True)
-    bufs = []
-    for sd in range(3):
-        nb = P.gpsv_batch_buffer_size_ext(ds, dl, d, du, dw, x, dims, sd, h)
-        bufs.append(torch.empty(max(nb, 1), dtype=torch.uint8, device="cuda") if nb else None)
-
-    def step():
-        for sd in range(3):
-            P.gpsv_batch(ds, dl, d, du, dw, x, dims, sd, bufs[sd], h, check=False)
-
-    for _ in range(warmup):
-        step()
+    x = b.clone()
+    adi = P.Adi(mode, dims, [coeffs], handle=h, max_refine=max_refine, tol=tol)
+    # warm-up graph of `warmup` steps, then the timed graph of `steps` steps
+    if warmup > 0:
+        adi.capture(x, warmup)
+        adi.replay(1)
+        torch.cuda.synchronize()
+    x.copy_(b)
+    adi.capture(x, steps)
     torch.cuda.synchronize()
+    if mode_name == "ir":
+        adi.refine_steps()  # reset the counter
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
-    for _ in range(steps):
-        step()
+    adi.replay(1)
     ev1.record(stream)
     torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
-    ev_ms = ev0.elapsed_time(ev1) / steps
-    status = h.last_status()
-    if status != 0:
-        raise RuntimeError(f"solve failed, status {status}")
-
-    # per-direction kernel durations (pass B)
-    per_dir = []
-    for sd in range(3):
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(steps)]
-        for a, z in evs:
-            a.record(stream)
-            P.gpsv_batch(ds, dl, d, du, dw, x, dims, sd, bufs[sd], h, check=False)
-            z.record(stream)
-        torch.cuda.synchronize()
-        per_dir.append(sum(a.elapsed_time(z) for a, z in evs) / steps)
-    kernels = [h.kernel_config(k) for k in range(3)]
-
-    # sampled-line parity at full size, each direction from a fresh RHS
-    from oracle import oracle as O
-    err = 0.0
-    for sd in range(3):
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ms = ev0.elapsed_time(ev1) / steps
+    refine = adi.refine_steps() if mode_name == "ir" else None
+    x.copy_(b)
+    dir_ms = adi.direction_ms(x, reps=3)
+    # one step from the initial field for the accuracy check
+    x.copy_(b)
+    adi.step(x, 1)
+    torch.cuda.synchronize()
+    if mode_name == "ir":
+        adi.refine_steps()  # reset, then count the checked step alone
         x.copy_(b)
-        P.gpsv_batch(ds, dl, d, du, dw, x, dims, sd, bufs[sd], h)
+        adi.step(x, 1)
+        refine_checked = adi.refine_steps()
+    else:
+        refine_checked = None
+    result = x.double().cpu().numpy()
+    cfg = [adi.config(k) for k in range(len(dims))]
+    adi.close()
+    del x
+    bpr = BYTES_PER_ROW[mode_name]
+    dom = int(np.argmax(dir_ms))
+    return {"ms_per_step": ms, "wall_ms_per_step": wall, "dir_ms": dir_ms, "dominant": dom,
+            "achieved_GBs": bpr * n / (dir_ms[dom] * 1e-3) / 1e9, "bytes_per_row": bpr,
+            "step_GBs": bpr * n * len(dims) / (ms * 1e-3) / 1e9,
+            "grows": len(dims) * n / (ms * 1e-3) / 1e9, "kernels": cfg,
+            "refine_steps_timed_max": refine, "refine_steps_checked": refine_checked,
+            "one_step": result}
+
+
+def dropin_direction_ms(P, torch, coeffs, b, dims, h, reps=3):
+    """In-place per-direction solves through pentadsolver_D_gpsv_batch."""
+    x = b.clone()
+    stream = torch.cuda.current_stream()
+    out = []
+    for sd in range(len(dims)):
+        nb = P.gpsv_batch_buffer_size_ext(*coeffs, x, dims, sd, h)
+        buf = torch.empty(nb, dtype=torch.uint8, device="cuda") if nb else None
+        P.gpsv_batch(*coeffs, x, dims, sd, buf, h)
         torch.cuda.synchronize()
-        lines = sample_lines(dims, sd, 32, 100 + sd)
-        idx = torch.from_numpy(lines.reshape(-1)).cuda()
-        got = x[idx].double().cpu().numpy().reshape(lines.shape)
-        coeffs = [a[idx].double().cpu().numpy().reshape(lines.shape) for a in (ds, dl, d, du, dw, b)]
-        for j in range(lines.shape[0]):
-            line = [np.ascontiguousarray(c[j]) for c in coeffs]
-            ref = O.solve_ld(*line, [dims[sd]], 0)
-            e = float(np.max(np.abs(got[j] - ref)) / np.max(np.abs(ref)))
-            err = max(err, e)
-    h.close()
-    del arrs, ds, dl, d, du, dw, b, x, bufs
-    torch.cuda.empty_cache()
-    return {"ms_per_step": ev_ms, "wall_ms_per_step": wall_ms, "dir_ms": per_dir,
-            "kernels": kernels, "max_rel_err": err, "elem_bytes": torch.finfo(dtype).bits // 8}
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            P.gpsv_batch(*coeffs, x, dims, sd, buf, h, check=False)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1) / reps)
+        del buf
+    return out, [h.kernel_config(k) for k in range(len(dims))]
 
 
-def cpu_baseline(dims, seed, iters=2):
+def oracle_step(coeffs_np, b_np, dims):
+    from oracle import oracle as O
+    x = b_np.copy()
+    for sd in range(len(dims)):
+        O.gpsv_batch(*coeffs_np, x, dims, sd)
+    return x
+
+
+def cpu_baseline(coeffs_np, b_np, dims, iters=2):
     """The reference's own OpenMP+AVX2 CPU solver (oracle/_ref, built from its
     sources) timed on this host, FP64, method of apps/adi_cpu.cpp:49-103."""
     from oracle import oracle as O
-    from pentadsolver_amd import workloads as W
     kind = "reference" if O.ref_available("avx") else "port"
-    arrs = W.dominant_np(dims, seed)
-    x = arrs[5].copy()
-    solve = (lambda sd: O.ref_gpsv_batch(*arrs[:5], x, dims, sd)) if kind == "reference" \
-        else (lambda sd: O.gpsv_batch(*arrs[:5], x, dims, sd))
-    for sd in range(3):  # warm-up, faults in every page
+    x = b_np.copy()
+
+    def solve(sd):
+        if kind == "reference":
+            O.ref_gpsv_batch(*coeffs_np, x, dims, sd)
+        else:
+            O.gpsv_batch(*coeffs_np, x, dims, sd)
+
+    for sd in range(3):  # warm-up step, faults in every page
         solve(sd)
     t = np.zeros(3)
     for _ in range(iters):
@@ -150,48 +164,87 @@ def cpu_baseline(dims, seed, iters=2):
     t /= iters
     rows = 3 * int(np.prod(dims))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": rows / t.sum() / 1e9, "unit": "GRows/s", "cores": threads, "kind": kind,
-            "sample": f"FP64 ADI step x/y/z on {dims[0]}x{dims[1]}x{dims[2]}, {iters} timed "
-                      f"steps after 1 warm-up, ms per direction "
-                      f"{t[0]*1e3:.1f}/{t[1]*1e3:.1f}/{t[2]*1e3:.1f}"}
+    return {"value": float(rows / t.sum() / 1e9), "unit": "GRows/s", "cores": threads,
+            "kind": kind,
+            "sample": f"FP64 ADI step x/y/z on {dims[0]}x{dims[1]}x{dims[2]} (same seeded "
+                      f"coefficients), {iters} timed steps after 1 warm-up step, ms per "
+                      f"direction {t[0]*1e3:.1f}/{t[1]*1e3:.1f}/{t[2]*1e3:.1f}, "
+                      f"OMP_NUM_THREADS={threads}"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--max-refine", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-13)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "the per-GPU path would shard whole lines across GPUs, see DESIGN.md",
-              file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); lines are "
+              "independent within a direction, so a multi-GPU run would shard whole lines "
+              "(see DESIGN.md, sharding)", file=sys.stderr)
         sys.exit(2)
 
     import torch
     P = import_pkg()
+    from pentadsolver_amd import workloads as W
     P.load_library()
     torch.cuda.set_device(0)
     dims = [args.size] * 3
     n = int(np.prod(dims))
-    rows = 3 * n
 
-    log(f"bench: fp64 ADI step {dims}")
-    r64 = run_mode(P, torch, dims, torch.float64, args.steps, args.warmup, seed=3)
-    log(f"bench: fp64 {r64['ms_per_step']:.3f} ms/step dirs {r64['dir_ms']} err {r64['max_rel_err']:.2e}")
-    log(f"bench: fp32 ADI step {dims}")
-    r32 = run_mode(P, torch, dims, torch.float32, args.steps, args.warmup, seed=3)
-    log(f"bench: fp32 {r32['ms_per_step']:.3f} ms/step dirs {r32['dir_ms']} err {r32['max_rel_err']:.2e}")
+    log(f"bench: generating {dims} coefficients (seed 3)")
+    ds, dl, d, du, dw, b = W.dominant_torch(dims, 3, dtype=torch.float64)
+    c64 = (ds, dl, d, du, dw)
+    c32 = tuple(a.float() for a in c64)
+    b32 = b.float()
+    h = P.Handle(stream=torch.cuda.current_stream())
 
-    head = r64
-    bpr = 7 * head["elem_bytes"]  # 5 coefficients + RHS read, solution written
-    dom = int(np.argmax(head["dir_ms"]))
-    achieved = bpr * n / (head["dir_ms"][dom] * 1e-3) / 1e9
+    res = {}
+    for mode, coeffs, rhs in (("ir", c32, b), ("fp64", c64, b), ("fp32", c32, b32)):
+        log(f"bench: {mode} ADI step")
+        res[mode] = time_adi(P, torch, mode, coeffs, rhs, dims, args.steps, args.warmup, h,
+                             args.max_refine, args.tol)
+        r = res[mode]
+        log(f"bench: {mode} {r['ms_per_step']:.3f} ms/step {r['grows']:.1f} GRows/s dirs "
+            f"{['%.3f' % v for v in r['dir_ms']]} kernels {r['kernels']} refine {r['refine_steps_timed_max']}")
+    log("bench: drop-in in-place FP64 direction solves")
+    dropin_ms, dropin_k = dropin_direction_ms(P, torch, c64, b, dims, h)
+
+    check = {}
+    c64_np = c32_np = b_np = None
+    if not args.no_check or not args.no_cpu:
+        c64_np = [a.cpu().numpy() for a in c64]
+        b_np = b.cpu().numpy()
+    if not args.no_check:
+        from oracle import oracle as O  # noqa: F401  (checker only)
+        log("bench: oracle FP64 step")
+        ref64 = oracle_step(c64_np, b_np, dims)
+        scale = np.max(np.abs(ref64))
+        check["fp64_max_rel_err"] = float(np.max(np.abs(res["fp64"]["one_step"] - ref64)) / scale)
+        del ref64
+        log("bench: oracle FP64 step of the FP32-stored coefficients")
+        c32_np = [a.cpu().numpy().astype(np.float64) for a in c32]
+        ref_w = oracle_step(c32_np, b_np, dims)
+        scale = np.max(np.abs(ref_w))
+        check["ir_max_rel_err"] = float(np.max(np.abs(res["ir"]["one_step"] - ref_w)) / scale)
+        ref32 = oracle_step(c32_np, b32.double().cpu().numpy(), dims)
+        check["fp32_max_rel_err"] = float(np.max(np.abs(res["fp32"]["one_step"] - ref32)) /
+                                          np.max(np.abs(ref32)))
+        del ref_w, ref32, c32_np
+        log(f"bench: errors {check}")
+    for r in res.values():
+        r.pop("one_step", None)
+
+    head = res["ir"]
+    dom = head["dominant"]
     result = {
         "metric": METRIC,
-        "value": rows / (head["ms_per_step"] * 1e-3) / 1e9,
+        "value": head["grows"],
         "unit": "GRows/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -200,27 +253,45 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 solve + f64 residual/update",
         "data": "synthetic",
-        "config": {"workload": f"ADI step x->y->z, {dims[0]}^3 grid, "
-                               f"{n // dims[0]} systems x {dims[0]} per direction, "
-                               "diagonally dominant seeded coefficients",
-                   "grid": dims, "mode": "plain FP64"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"dir {dom}: {head['kernels'][dom]}",
-                     "bytes_per_row": bpr},
-        "max_rel_err_vs_oracle": head["max_rel_err"],
+        "config": {"workload": f"ADI step x->y->z on a {dims[0]}x{dims[1]}x{dims[2]} grid, "
+                               f"{n // dims[0]} systems of length {dims[0]} per direction, "
+                               "FP32-stored diagonally dominant coefficients (ratio U(2,3)), "
+                               "FP64 field, iterative refinement to tol "
+                               f"{args.tol:g} (max {args.max_refine} steps), hipGraph replay",
+                   "grid": dims, "mode": "FP32 + FP64 refinement"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "kernel": f"direction {dom}: {head['kernels'][dom]}",
+                     "bytes_per_row": head["bytes_per_row"],
+                     "rows_per_launch": n, "kernel_ms": head["dir_ms"][dom]},
+        "max_rel_err_vs_fp64_oracle": check.get("ir_max_rel_err"),
+        "refine_steps": {"timed_max": head["refine_steps_timed_max"],
+                         "one_step_from_initial_field": head["refine_steps_checked"]},
         "per_direction_ms": head["dir_ms"],
         "kernels": head["kernels"],
+        "step_GBs": head["step_GBs"],
         "wall_ms_per_step": head["wall_ms_per_step"],
-        "fp32": {"value": rows / (r32["ms_per_step"] * 1e-3) / 1e9,
-                 "ms_per_step": r32["ms_per_step"], "per_direction_ms": r32["dir_ms"],
-                 "kernels": r32["kernels"], "max_rel_err_vs_fp64_oracle": r32["max_rel_err"]},
+        "fp64": {"value": res["fp64"]["grows"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "per_direction_ms": res["fp64"]["dir_ms"],
+                 "roofline_frac": res["fp64"]["achieved_GBs"] / HBM_PEAK_GBS,
+                 "kernels": res["fp64"]["kernels"],
+                 "max_rel_err_vs_fp64_oracle": check.get("fp64_max_rel_err")},
+        "fp32": {"value": res["fp32"]["grows"], "ms_per_step": res["fp32"]["ms_per_step"],
+                 "per_direction_ms": res["fp32"]["dir_ms"],
+                 "roofline_frac": res["fp32"]["achieved_GBs"] / HBM_PEAK_GBS,
+                 "kernels": res["fp32"]["kernels"],
+                 "max_rel_err_vs_fp64_oracle": check.get("fp32_max_rel_err")},
+        "ir_speedup_vs_fp64": res["fp64"]["ms_per_step"] / head["ms_per_step"],
+        "dropin_fp64_inplace": {"per_direction_ms": dropin_ms, "kernels": dropin_k,
+                                "grows": 3 * n / (sum(dropin_ms) * 1e-3) / 1e9},
     }
     if not args.no_cpu:
-        log("bench: cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(dims, seed=3)
+        log("bench: cpu baseline (reference CPU solver)")
+        result["cpu_baseline"] = cpu_baseline(c64_np, b_np, dims)
+    h.close()
     print(json.dumps(result), flush=True)
 
 

@@ -140,6 +140,116 @@ pentadStatus_t pentadsolver_freeze_env(pentadsolver_handle_t handle,
 pentadStatus_t pentadsolver_set_kernel_params(pentadsolver_handle_t handle,
                                               int dir, int rows, int width);
 
+/* Mixed precision: coefficients in FP32, right-hand side and solution in
+ * FP64.  The bulk solve runs in FP32; the residual r = b - A x (A widened
+ * exactly to FP64) and the update x += dx are the only FP64 steps, and the
+ * refinement of each system stops once the estimated relative forward error
+ * of the iterate, max|dx_k|^2 / (max|dx_{k-1}| * max|x|) (dx_0 = the FP32
+ * solution), is at most tol, or after max_refine corrections.  The result solves widen(A) x = b to FP64
+ * accuracy for well-conditioned systems.  Factorisation, residuals and
+ * updates stay on chip: HBM traffic is 20 B of coefficients + 8 B in + 8 B
+ * out per row.  d_refine_steps (device int*, may be NULL) receives
+ * atomicMax(*d_refine_steps, steps taken); the caller zeroes it.  t_buffer
+ * as for pentadsolver_D_gpsv_batch, sized by the _ir_buffer_size_ext call.
+ * No reference counterpart (the reference solves in one precision). */
+size_t pentadsolver_DS_gpsv_batch_ir_buffer_size_ext(
+    pentadsolver_handle_t handle, const float *ds, const float *dl,
+    const float *d, const float *du, const float *dw, const double *x,
+    const int *t_dims, int t_ndim, int t_solvedim);
+void pentadsolver_DS_gpsv_batch_ir(pentadsolver_handle_t handle,
+                                   const float *ds, const float *dl,
+                                   const float *d, const float *du,
+                                   const float *dw, double *x,
+                                   const int *t_dims, int t_ndim,
+                                   int t_solvedim, int max_refine, double tol,
+                                   int *d_refine_steps, void *t_buffer);
+
+/* Autotuner: times every kernel configuration that can solve this shape
+ * (rows per lane x lanes per system for x; rows per lane x systems per
+ * workgroup for y/z; the one-lane-per-system kernel when t_buffer holds its
+ * scratch), reps launches each on the handle's stream, and stores the
+ * fastest in the handle's table for (precision, direction, length).  x is
+ * restored before returning.  Synchronises the stream and allocates a
+ * temporary copy of x.  Returns the number of configurations timed (< 0 on
+ * error); report (may be NULL) receives one "config time" line each.
+ * No reference counterpart (the reference uses fixed dispatch rules and
+ * PENTA_PCR_LANES, src/cuda/singlenode/pentad_cuda.cu:613-697). */
+int pentadsolver_D_autotune(pentadsolver_handle_t handle, const double *ds,
+                            const double *dl, const double *d,
+                            const double *du, const double *dw, double *x,
+                            const int *t_dims, int t_ndim, int t_solvedim,
+                            void *t_buffer, size_t buffer_bytes, int reps,
+                            char *report, size_t report_len);
+int pentadsolver_S_autotune(pentadsolver_handle_t handle, const float *ds,
+                            const float *dl, const float *d, const float *du,
+                            const float *dw, float *x, const int *t_dims,
+                            int t_ndim, int t_solvedim, void *t_buffer,
+                            size_t buffer_bytes, int reps, char *report,
+                            size_t report_len);
+int pentadsolver_DS_autotune_ir(pentadsolver_handle_t handle, const float *ds,
+                                const float *dl, const float *d,
+                                const float *du, const float *dw, double *x,
+                                const int *t_dims, int t_ndim, int t_solvedim,
+                                int max_refine, double tol, void *t_buffer,
+                                size_t buffer_bytes, int reps, char *report,
+                                size_t report_len);
+void pentadsolver_clear_tuning(pentadsolver_handle_t handle);
+
+/* Bandwidth probe with the solve's access pattern: out[i] = in0[i] + ... +
+ * in5[i] followed by `fmas` FMAs, n elements of elem_bytes (4 or 8), on the
+ * handle's stream.  Counterpart of the reference's
+ * apps/roofline_probe_cuda.cu:27-49 (there a stand-alone app). */
+pentadStatus_t pentadsolver_probe_stream(pentadsolver_handle_t handle,
+                                         const void *const in[6], void *out,
+                                         size_t n, int elem_bytes, int fmas);
+
+/* ADI sweep driver.  One step solves along every direction in turn (x, then
+ * y, then z for ndim = 3) in place on a field x in the standard layout
+ * (t_dims[0] fastest), the loop of the reference's apps/adi_cuda.cu:167-176.
+ * The driver keeps its own copies of the coefficients, laid out once per
+ * direction so that every direction reads contiguous lines and writes its
+ * solution in the layout the next direction reads; between the steps of one
+ * call x holds the field in an internal layout, and after the last step it
+ * is back in the caller's layout.  x must be 16-byte aligned.
+ *   mode 0: FP64 coefficients and field; mode 1: FP32 both;
+ *   mode 2: FP32 coefficients, FP64 field, FP32 solve + FP64 refinement
+ *           (max_refine, tol as in pentadsolver_DS_gpsv_batch_ir).
+ *   coeffs: coeff_sets * 5 device pointers (ds, dl, d, du, dw per set) in
+ *   the standard layout with elements of coeff_elem_bytes (4 or 8); one set
+ *   serves every direction (as the reference's apps do), or ndim sets give
+ *   each direction its own.
+ * create allocates (coefficients: 5 * ndim copies; two field-sized
+ * buffers) and synchronises the handle's stream.  No reference counterpart
+ * as a library call. */
+typedef void *pentadsolver_adi_t;
+pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t handle,
+                                       pentadsolver_adi_t *adi, int mode,
+                                       int ndim, const int *dims,
+                                       const void *const *coeffs,
+                                       int coeff_sets, int coeff_elem_bytes,
+                                       int max_refine, double tol);
+/* nsteps ADI steps on x, launched on the handle's stream (asynchronous). */
+pentadStatus_t pentadsolver_adi_step(pentadsolver_adi_t adi, void *x,
+                                     int nsteps);
+/* Capture nsteps steps on x as one hipGraph (stream capture of the same
+ * launches); pentadsolver_adi_graph_launch replays the whole capture reps
+ * times on the handle's stream.  Capture again to change x or nsteps. */
+pentadStatus_t pentadsolver_adi_graph_capture(pentadsolver_adi_t adi, void *x,
+                                              int nsteps);
+pentadStatus_t pentadsolver_adi_graph_launch(pentadsolver_adi_t adi, int reps);
+/* reps + 2 steps on x with HIP events around each direction's launch on
+ * the handle's stream (synchronises); ms[d] = mean duration of direction d
+ * over the reps steady-state steps.  The per-direction pass of the
+ * reference's apps/adi_cuda.cu:216-256. */
+pentadStatus_t pentadsolver_adi_direction_ms(pentadsolver_adi_t adi, void *x,
+                                             int reps, float *ms);
+/* Largest number of refinement steps any line took since the last call
+ * (mode 2; synchronises the stream), then resets the counter. */
+int pentadsolver_adi_refine_steps(pentadsolver_adi_t adi);
+/* Kernel configuration used along direction dir. */
+const char *pentadsolver_adi_config(pentadsolver_adi_t adi, int dir);
+void pentadsolver_adi_destroy(pentadsolver_adi_t *adi);
+
 /* Name and tuning parameters of the kernel the last solve of this handle
  * along dir launched, e.g. "thomas-pcr rows=8 lanes=32". */
 const char *pentadsolver_kernel_config(pentadsolver_handle_t handle, int dir);

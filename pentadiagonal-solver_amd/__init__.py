@@ -44,6 +44,21 @@ _SIGNATURES = {
     "pentadsolver_freeze_env": (_I, [_P, _I]),
     "pentadsolver_kernel_config": (ctypes.c_char_p, [_P, _I]),
     "pentadsolver_set_kernel_params": (_I, [_P, _I, _I, _I]),
+    "pentadsolver_DS_gpsv_batch_ir_buffer_size_ext": (_SZ, [_P] * 7 + [_IP, _I, _I]),
+    "pentadsolver_DS_gpsv_batch_ir": (None, [_P] * 7 + [_IP, _I, _I, _I, ctypes.c_double, _P, _P]),
+    "pentadsolver_D_autotune": (_I, [_P] * 7 + [_IP, _I, _I, _P, _SZ, _I, ctypes.c_char_p, _SZ]),
+    "pentadsolver_S_autotune": (_I, [_P] * 7 + [_IP, _I, _I, _P, _SZ, _I, ctypes.c_char_p, _SZ]),
+    "pentadsolver_DS_autotune_ir": (_I, [_P] * 7 + [_IP, _I, _I, _I, ctypes.c_double, _P, _SZ, _I, ctypes.c_char_p, _SZ]),
+    "pentadsolver_clear_tuning": (None, [_P]),
+    "pentadsolver_probe_stream": (_I, [ctypes.POINTER(_P), _P, _SZ, _I, _I]),
+    "pentadsolver_adi_create": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), _I, _I, _I, ctypes.c_double]),
+    "pentadsolver_adi_step": (_I, [_P, _P, _I]),
+    "pentadsolver_adi_graph_capture": (_I, [_P, _P, _I]),
+    "pentadsolver_adi_graph_launch": (_I, [_P, _I]),
+    "pentadsolver_adi_refine_steps": (_I, [_P]),
+    "pentadsolver_adi_direction_ms": (_I, [_P, _P, _I, ctypes.POINTER(ctypes.c_float)]),
+    "pentadsolver_adi_config": (ctypes.c_char_p, [_P, _I]),
+    "pentadsolver_adi_destroy": (None, [ctypes.POINTER(_P)]),
 }
 
 STATUS_NAMES = {
@@ -204,6 +219,149 @@ def solve(ds, dl, d, du, dw, x, dims: Sequence[int], solvedim: int, handle=None)
         buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
     gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, buf, handle)
     return buf
+
+
+def gpsv_batch_ir_buffer_size_ext(ds, dl, d, du, dw, x, dims: Sequence[int],
+                                  solvedim: int, handle=None) -> int:
+    """Scratch bytes of the mixed-precision solve (0 for the on-chip kernels)."""
+    return int(load_library().pentadsolver_DS_gpsv_batch_ir_buffer_size_ext(
+        _hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+        _dims_arg(dims), len(dims), int(solvedim)))
+
+
+def gpsv_batch_ir(ds, dl, d, du, dw, x, dims: Sequence[int], solvedim: int,
+                  max_refine: int = 5, tol: float = 1e-13, steps=None,
+                  buffer=None, handle=None, check: bool = True) -> None:
+    """FP32 solve + FP64 iterative refinement, in place on the float64 x.
+
+    ds..dw: float32 device tensors; x: float64 device tensor (RHS in,
+    solution out); steps: optional int32 device tensor (1 element) that
+    receives the largest number of refinement steps taken.
+    """
+    import torch
+    for t in (ds, dl, d, du, dw):
+        if t.dtype != torch.float32:
+            raise TypeError("coefficients must be float32")
+    if x.dtype != torch.float64:
+        raise TypeError("x must be float64")
+    lib = load_library()
+    lib.pentadsolver_DS_gpsv_batch_ir(
+        _hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+        _dims_arg(dims), len(dims), int(solvedim), int(max_refine), float(tol),
+        _ptr(steps), _ptr(buffer))
+    if check:
+        _check(lib.pentadsolver_get_last_status(_hval(handle)))
+
+
+def solve_ir(ds, dl, d, du, dw, x, dims, solvedim, max_refine=5, tol=1e-13,
+             steps=None, handle=None):
+    """gpsv_batch_ir with the scratch buffer allocated as needed."""
+    import torch
+    nbytes = gpsv_batch_ir_buffer_size_ext(ds, dl, d, du, dw, x, dims, solvedim, handle)
+    buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device) if nbytes else None
+    gpsv_batch_ir(ds, dl, d, du, dw, x, dims, solvedim, max_refine, tol, steps, buf, handle)
+    return buf
+
+
+def autotune(ds, dl, d, du, dw, x, dims, solvedim, handle, reps=3, buffer=None,
+             ir=False, max_refine=5, tol=1e-13):
+    """Time every configuration for this shape; the handle keeps the fastest.
+
+    Returns the list of (config, ms) the library measured.  x is unchanged.
+    """
+    lib = load_library()
+    rep = ctypes.create_string_buffer(1 << 14)
+    nb = 0 if buffer is None else buffer.numel() * buffer.element_size()
+    dd = _dims_arg(dims)
+    args = [_ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x), dd, len(dims), int(solvedim)]
+    if ir:
+        n = lib.pentadsolver_DS_autotune_ir(_hval(handle), *args, int(max_refine), float(tol),
+                                            _ptr(buffer), nb, int(reps), rep, len(rep))
+    else:
+        fn = getattr(lib, f"pentadsolver_{_prefix(x.dtype)}_autotune")
+        n = fn(_hval(handle), *args, _ptr(buffer), nb, int(reps), rep, len(rep))
+    if n < 0:
+        raise PentadError("autotune failed")
+    out = []
+    for line in rep.value.decode().splitlines():
+        cfg, ms, _ = line.rsplit(" ", 2)
+        out.append((cfg, float(ms)))
+    return out
+
+
+def probe_stream(ins, out, fmas=0, handle=None):
+    """Launch the 6-read/1-write bandwidth probe on device tensors."""
+    arr = (_P * 6)(*[_ptr(t) for t in ins])
+    _check(load_library().pentadsolver_probe_stream(
+        _hval(handle), arr, _ptr(out), out.numel(), out.element_size(), int(fmas)))
+
+
+ADI_FP64, ADI_FP32, ADI_IR = 0, 1, 2
+
+
+class Adi:
+    """ADI sweep driver (pentadsolver_adi_*): x, then y, then z in place.
+
+    mode: ADI_FP64 (float64 field and coefficients), ADI_FP32 (float32),
+    ADI_IR (float32 coefficients solved in FP32 with FP64 refinement on a
+    float64 field).  coeff_sets: one (ds, dl, d, du, dw) tuple of device
+    tensors for every direction, or one tuple per direction; float32 or
+    float64 (converted as the mode needs).
+    """
+
+    def __init__(self, mode, dims, coeff_sets, handle=None, max_refine=5, tol=1e-13):
+        lib = load_library()
+        self._a = _P()
+        self.ndim = len(dims)
+        flat = [t for cs in coeff_sets for t in cs]
+        eb = flat[0].element_size()
+        if any(t.element_size() != eb for t in flat):
+            raise TypeError("coefficient tensors must share one dtype")
+        ptrs = (_P * len(flat))(*[_ptr(t) for t in flat])
+        self._keep = flat
+        _check(lib.pentadsolver_adi_create(_hval(handle), ctypes.byref(self._a), int(mode),
+                                           len(dims), _dims_arg(dims), ptrs, len(coeff_sets),
+                                           eb, int(max_refine), float(tol)))
+        self._keep = None  # the driver holds its own copies
+
+    def step(self, x, nsteps=1):
+        _check(load_library().pentadsolver_adi_step(self._a, _ptr(x), int(nsteps)))
+
+    def capture(self, x, nsteps=1):
+        """Capture nsteps steps on x as one hipGraph."""
+        _check(load_library().pentadsolver_adi_graph_capture(self._a, _ptr(x), int(nsteps)))
+
+    def replay(self, reps=1):
+        """Replay the captured steps reps times."""
+        _check(load_library().pentadsolver_adi_graph_launch(self._a, int(reps)))
+
+    def direction_ms(self, x, reps=5):
+        ms = (ctypes.c_float * 3)()
+        _check(load_library().pentadsolver_adi_direction_ms(self._a, _ptr(x), int(reps), ms))
+        return [float(v) for v in ms[: self.ndim]]
+
+    def refine_steps(self) -> int:
+        return load_library().pentadsolver_adi_refine_steps(self._a)
+
+    def config(self, direction: int) -> str:
+        return load_library().pentadsolver_adi_config(self._a, direction).decode()
+
+    def close(self):
+        if self._a:
+            load_library().pentadsolver_adi_destroy(ctypes.byref(self._a))
+            self._a = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def kernel_that_ran(direction: int) -> str:

@@ -1,0 +1,344 @@
+// ir_core.hpp -- FP32 solve + FP64 iterative refinement, fused per line.
+//
+// Mixed-precision iterative refinement on the partitioned layout of
+// partition_core.hpp (same chunks, separators and 2x2 block reduced system):
+//
+//   factor A (FP32) once: interior LU, separator spikes, and the block
+//                         PCR multipliers of every round
+//   x0  = solve32(A, fp32(b))
+//   repeat:  r  = b - A x      (FP64, A widened exactly from FP32)
+//            dx = solve32(A, fp32(r))
+//            x += dx           (FP64)
+//   until the estimated relative forward error of x,
+//   max|dx_k|^2 / (max|dx_{k-1}| max|x|) with dx_0 = x0, is <= tol
+//   (maxima over the system), or max_steps corrections were made.
+//
+// The residual and the update are the only FP64 arithmetic.  Coefficients
+// are read once from HBM in FP32 (20 B/row) and the RHS/solution in FP64
+// (8 + 8 B/row); every refinement step runs on chip.  The FP64 result solves
+// widen(A32) x = b, the system the FP64 oracle solves.
+//
+// Register budget: the coefficients are reached through an accessor (Cf),
+// which the x-direction kernel backs with LDS so that the per-lane state is
+// the FP64 iterate, the LU factors (du2, 1/pivot, two spikes) and the PCR
+// multipliers; the other two factors (the eliminated lower coupling and
+// du2's partner dw2) are recomputed from the coefficients when used.  The
+// right-hand side is reached through Bv (registers, or re-read from memory).
+//
+// No reference counterpart (the reference solves in one precision only).
+#pragma once
+
+#include "partition_core.hpp"
+#include "partition_exchange.hpp"
+
+namespace pentad {
+
+// coefficient accessors: s(i), l(i), d(i), u(i), w(i) for chunk row i
+template <int M> struct RegCoef {
+  const float (&s_)[M], (&l_)[M], (&d_)[M], (&u_)[M], (&w_)[M];
+  __device__ __forceinline__ float s(int i) const { return s_[i]; }
+  __device__ __forceinline__ float l(int i) const { return l_[i]; }
+  __device__ __forceinline__ float d(int i) const { return d_[i]; }
+  __device__ __forceinline__ float u(int i) const { return u_[i]; }
+  __device__ __forceinline__ float w(int i) const { return w_[i]; }
+  __device__ __forceinline__ void refresh() const {}
+};
+
+// LDS layout [5][M][64] per wavefront: lane-minor, conflict-free.
+template <int M> struct LdsCoef {
+  const float *base; // this wavefront's region + lane
+  __device__ __forceinline__ float at(int k, int i) const { return base[(k * M + i) * 64]; }
+  __device__ __forceinline__ float s(int i) const { return at(0, i); }
+  __device__ __forceinline__ float l(int i) const { return at(1, i); }
+  __device__ __forceinline__ float d(int i) const { return at(2, i); }
+  __device__ __forceinline__ float u(int i) const { return at(3, i); }
+  __device__ __forceinline__ float w(int i) const { return at(4, i); }
+  // Forget earlier loads (compiler-level memory clobber, no instruction):
+  // without it the compiler keeps every coefficient it has read in
+  // registers, which is what keeping them in LDS is meant to avoid.
+  __device__ __forceinline__ void refresh() const { asm volatile("" ::: "memory"); }
+};
+
+// right-hand side accessors
+template <int M> struct RegRhs {
+  const double (&b_)[M];
+  __device__ __forceinline__ double operator()(int i) const { return b_[i]; }
+};
+struct MemRhs { // rows i of this lane's chunk: p[i * stride]
+  const double *p;
+  int64_t stride;
+  __device__ __forceinline__ double operator()(int i) const { return p[i * stride]; }
+};
+
+// Factorisation of one chunk: du2, 1/pivot, the previous-separator spikes
+// (za, zb), the separator couplings and the block PCR multipliers of each
+// round (MAXR >= ceil(log2 P)).
+template <int M, int MAXR> struct IrFactors {
+  static constexpr int m = M - 2;
+  float du2[m], iv[m], za[m], zb[m];
+  float sa, la, wa, sb, ub, wb;
+  M2<float> Bi0;
+  M2<float> Ah[MAXR], Ch[MAXR], Bi[MAXR];
+};
+
+// eliminated lower coupling of interior row i (ddl) and dw2 = w / pivot
+template <int M, int MAXR, typename Cf>
+__device__ __forceinline__ float ir_ddl(const IrFactors<M, MAXR> &F, const Cf &A, int i) {
+  return (i == 1) ? A.l(1) : A.l(i) - A.s(i) * F.du2[i - 2];
+}
+template <int M, int MAXR, typename Cf>
+__device__ __forceinline__ float ir_dw2(const IrFactors<M, MAXR> &F, const Cf &A, int i) {
+  return A.w(i) * F.iv[i];
+}
+
+template <int M, int MAXR, typename Cf, typename Ex>
+__device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, int p,
+                                          int P, Ex &ex) {
+  constexpr int m = M - 2;
+  const bool first = (p == 0), last = (p == P - 1);
+  float dw2[m]; // kept only during the factorisation
+  F.iv[0] = recip(A.d(0));
+  F.du2[0] = A.u(0) * F.iv[0];
+  dw2[0] = A.w(0) * F.iv[0];
+  F.za[0] = first ? 0.0f : -A.s(0) * F.iv[0];
+  F.zb[0] = first ? 0.0f : -A.l(0) * F.iv[0];
+  {
+    const float l1 = A.l(1);
+    F.iv[1] = recip(A.d(1) - l1 * F.du2[0]);
+    F.du2[1] = (A.u(1) - l1 * dw2[0]) * F.iv[1];
+    dw2[1] = A.w(1) * F.iv[1];
+    F.za[1] = (-l1 * F.za[0]) * F.iv[1];
+    F.zb[1] = ((first ? 0.0f : -A.s(1)) - l1 * F.zb[0]) * F.iv[1];
+  }
+#pragma unroll
+  for (int i = 2; i < m; ++i) {
+    const float si = A.s(i);
+    const float dl_ = A.l(i) - si * F.du2[i - 2];
+    F.iv[i] = recip(A.d(i) - si * dw2[i - 2] - dl_ * F.du2[i - 1]);
+    F.du2[i] = (A.u(i) - dl_ * dw2[i - 1]) * F.iv[i];
+    dw2[i] = A.w(i) * F.iv[i];
+    F.za[i] = (-si * F.za[i - 2] - dl_ * F.za[i - 1]) * F.iv[i];
+    F.zb[i] = (-si * F.zb[i - 2] - dl_ * F.zb[i - 1]) * F.iv[i];
+  }
+  // backward sweep of the spike components [s0, s1, t0, t1]
+  float XL1[4], XL2[4], XF0[4], XF1[4];
+  XL1[0] = F.za[m - 1];
+  XL1[1] = F.zb[m - 1];
+  XL1[2] = -F.du2[m - 1];
+  XL1[3] = -dw2[m - 1];
+  XL2[0] = F.za[m - 2] - F.du2[m - 2] * XL1[0];
+  XL2[1] = F.zb[m - 2] - F.du2[m - 2] * XL1[1];
+  XL2[2] = -F.du2[m - 2] * XL1[2] - dw2[m - 2];
+  XL2[3] = -F.du2[m - 2] * XL1[3];
+  {
+    float xa[4], xb[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      xa[c] = XL2[c];
+      xb[c] = XL1[c];
+    }
+#pragma unroll
+    for (int i = m - 3; i >= 0; --i) {
+      const float z2[2] = {F.za[i], F.zb[i]};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float zc = (c < 2) ? z2[c] : 0.0f;
+        const float xi = zc - F.du2[i] * xa[c] - dw2[i] * xb[c];
+        xb[c] = xa[c];
+        xa[c] = xi;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      XF0[c] = xa[c];
+      XF1[c] = xb[c];
+    }
+  }
+  float Y0[4], Y1[4];
+  {
+    float send[8], rp[8], rn[8];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      send[c] = XF0[c];
+      send[4 + c] = XF1[c];
+    }
+    ex.template swap<8>(send, rp, rn, 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      Y0[c] = rn[c];
+      Y1[c] = rn[4 + c];
+    }
+  }
+  F.sa = A.s(m);
+  F.la = A.l(m);
+  F.wa = last ? 0.0f : A.w(m);
+  F.sb = A.s(m + 1);
+  F.ub = last ? 0.0f : A.u(m + 1);
+  F.wb = last ? 0.0f : A.w(m + 1);
+  const float da = A.d(m), ua = A.u(m), lb = A.l(m + 1), db = A.d(m + 1);
+  M2<float> Am, B, C;
+  Am.a00 = F.sa * XL2[0] + F.la * XL1[0];
+  Am.a01 = F.sa * XL2[1] + F.la * XL1[1];
+  B.a00 = F.sa * XL2[2] + F.la * XL1[2] + da + F.wa * Y0[0];
+  B.a01 = F.sa * XL2[3] + F.la * XL1[3] + ua + F.wa * Y0[1];
+  C.a00 = F.wa * Y0[2];
+  C.a01 = F.wa * Y0[3];
+  Am.a10 = F.sb * XL1[0];
+  Am.a11 = F.sb * XL1[1];
+  B.a10 = F.sb * XL1[2] + lb + F.ub * Y0[0] + F.wb * Y1[0];
+  B.a11 = F.sb * XL1[3] + db + F.ub * Y0[1] + F.wb * Y1[1];
+  C.a10 = F.ub * Y0[2] + F.wb * Y1[2];
+  C.a11 = F.ub * Y0[3] + F.wb * Y1[3];
+  F.Bi0 = inv(B);
+  Am = mul(F.Bi0, Am);
+  C = mul(F.Bi0, C);
+#pragma unroll
+  for (int j = 0; j < MAXR; ++j) {
+    const int h = 1 << j;
+    if (h < P) {
+      float send[8] = {Am.a00, Am.a01, Am.a10, Am.a11, C.a00, C.a01, C.a10, C.a11};
+      float rp[8], rn[8];
+      ex.template swap<8>(send, rp, rn, h);
+      const M2<float> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
+      const M2<float> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
+      F.Ah[j] = Am;
+      F.Ch[j] = C;
+      const M2<float> ACp = mul(Am, Cp), CAn = mul(C, An);
+      const M2<float> Bn{1.0f - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
+                         -ACp.a10 - CAn.a10, 1.0f - ACp.a11 - CAn.a11};
+      F.Bi[j] = inv(Bn);
+      const M2<float> AA = mul(Am, Ap), CC = mul(C, Cn);
+      Am = mul(F.Bi[j], M2<float>{-AA.a00, -AA.a01, -AA.a10, -AA.a11});
+      C = mul(F.Bi[j], M2<float>{-CC.a00, -CC.a01, -CC.a10, -CC.a11});
+    }
+  }
+}
+
+// Solve A y = f with the factors; f (M rows of this chunk) is overwritten by y.
+template <int M, int MAXR, typename Cf, typename Ex>
+__device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &A,
+                                         float (&f)[M], int p, int P, Ex &ex) {
+  constexpr int m = M - 2;
+  A.refresh();
+  float zf[m];
+  zf[0] = f[0] * F.iv[0];
+  zf[1] = (f[1] - A.l(1) * zf[0]) * F.iv[1];
+#pragma unroll
+  for (int i = 2; i < m; ++i)
+    zf[i] = (f[i] - A.s(i) * zf[i - 2] - ir_ddl(F, A, i) * zf[i - 1]) * F.iv[i];
+  // f-component of the interior's first and last two rows (separators = 0)
+  const float xl1 = zf[m - 1];
+  const float xl2 = zf[m - 2] - F.du2[m - 2] * xl1;
+  float xa = xl2, xb = xl1;
+#pragma unroll
+  for (int i = m - 3; i >= 0; --i) {
+    const float xi = zf[i] - F.du2[i] * xa - ir_dw2(F, A, i) * xb;
+    xb = xa;
+    xa = xi;
+  }
+  float y0, y1;
+  {
+    float send[2] = {xa, xb}, rp[2], rn[2];
+    ex.template swap<2>(send, rp, rn, 1);
+    y0 = rn[0];
+    y1 = rn[1];
+  }
+  const float r0 = f[m] - F.sa * xl2 - F.la * xl1 - F.wa * y0;
+  const float r1 = f[m + 1] - F.sb * xl1 - F.ub * y0 - F.wb * y1;
+  float q0 = F.Bi0.a00 * r0 + F.Bi0.a01 * r1;
+  float q1 = F.Bi0.a10 * r0 + F.Bi0.a11 * r1;
+#pragma unroll
+  for (int j = 0; j < MAXR; ++j) {
+    const int h = 1 << j;
+    if (h < P) {
+      float send[2] = {q0, q1}, rp[2], rn[2];
+      ex.template swap<2>(send, rp, rn, h);
+      const M2<float> &Ah = F.Ah[j], &Ch = F.Ch[j], &Bi = F.Bi[j];
+      const float t0 = q0 - (Ah.a00 * rp[0] + Ah.a01 * rp[1]) - (Ch.a00 * rn[0] + Ch.a01 * rn[1]);
+      const float t1 = q1 - (Ah.a10 * rp[0] + Ah.a11 * rp[1]) - (Ch.a10 * rn[0] + Ch.a11 * rn[1]);
+      q0 = Bi.a00 * t0 + Bi.a01 * t1;
+      q1 = Bi.a10 * t0 + Bi.a11 * t1;
+    }
+  }
+  float s0, s1;
+  {
+    float send[2] = {q0, q1}, rp[2], rn[2];
+    ex.template swap<2>(send, rp, rn, 1);
+    s0 = rp[0];
+    s1 = rp[1];
+  }
+  float xn1 = q0, xn2 = q1;
+  A.refresh();
+#pragma unroll
+  for (int i = m - 1; i >= 0; --i) {
+    const float hi = zf[i] + F.za[i] * s0 + F.zb[i] * s1;
+    const float xi = hi - F.du2[i] * xn1 - ir_dw2(F, A, i) * xn2;
+    f[i] = xi;
+    xn2 = xn1;
+    xn1 = xi;
+  }
+  f[m] = q0;
+  f[m + 1] = q1;
+}
+
+// Full refinement on one chunk.  b: FP64 RHS rows (accessor); x: FP64
+// solution rows (output).  Returns the number of refinement steps taken
+// (uniform over the lanes that exchange with each other).
+template <int M, int MAXR, typename Cf, typename Bv, typename Ex>
+__device__ __forceinline__ int ir_solve(const Cf &A, const Bv &b, double (&x)[M],
+                                        int p, int P, Ex &ex, int max_steps,
+                                        double tol) {
+  const bool first = (p == 0), last = (p == P - 1);
+  IrFactors<M, MAXR> F;
+  ir_factor<M, MAXR>(F, A, p, P, ex);
+  float f[M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) f[i] = (float)b(i);
+  ir_apply<M, MAXR>(F, A, f, p, P, ex);
+  double xloc = 0.0;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    x[i] = (double)f[i];
+    xloc = fmax(xloc, fabs(x[i]));
+  }
+  const double xnorm = max_all(ex, xloc, P);
+  double dprev = xnorm; // dx_0 := x_0
+  int k = 0;
+  while (k < max_steps) {
+    // halo: the previous chunk's last two rows, the next chunk's first two
+    double send[4] = {x[0], x[1], x[M - 2], x[M - 1]}, hp[4], hn[4];
+    ex.template swap<4>(send, hp, hn, 1);
+    double dxm = 0.0;
+    A.refresh();
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      const double xm2 = (i >= 2) ? x[i - 2] : hp[2 + i];
+      const double xm1 = (i >= 1) ? x[i - 1] : hp[3];
+      const double xp1 = (i + 1 < M) ? x[i + 1] : hn[0];
+      const double xp2 = (i + 2 < M) ? x[i + 2] : hn[i + 2 - M];
+      double acc = (double)A.d(i) * x[i];
+      acc += (first && i < 2) ? 0.0 : (double)A.s(i) * xm2;
+      acc += (first && i < 1) ? 0.0 : (double)A.l(i) * xm1;
+      acc += (last && i >= M - 1) ? 0.0 : (double)A.u(i) * xp1;
+      acc += (last && i >= M - 2) ? 0.0 : (double)A.w(i) * xp2;
+      f[i] = (float)(b(i) - acc);
+    }
+    ir_apply<M, MAXR>(F, A, f, p, P, ex);
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      x[i] += (double)f[i];
+      dxm = fmax(dxm, fabs((double)f[i]));
+    }
+    ++k;
+    // Stop once the estimated relative forward error of the new iterate,
+    // |dx_k|^2 / (|dx_{k-1}| |x|) (contraction |dx_k|/|dx_{k-1}| times the
+    // error |dx_k| it removed), is at most tol; norms are system maxima.
+    const double dxs = max_all(ex, dxm, P);
+    const bool conv = (dxs == 0.0) || (dxs * dxs <= tol * xnorm * dprev);
+    dprev = dxs;
+    if (ex.all(conv)) break;
+  }
+  return k;
+}
+
+} // namespace pentad

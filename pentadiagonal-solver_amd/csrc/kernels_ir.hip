@@ -1,0 +1,330 @@
+// kernels_ir.hip -- FP32 + FP64-refinement batched solves for gfx950.
+//
+// k_ir_x    x direction: L lanes of a wavefront per system, M rows per lane,
+//           FP32 coefficients via 16-byte loads, FP64 RHS/solution via
+//           16-byte loads.  HBM traffic per row: 5 x 4 B + 8 B in, 8 B out.
+// k_ir_yz   strided directions: S systems x P chunks per workgroup, LDS
+//           exchange, XCD-aware tile order so that workgroups reading the
+//           same 128-byte lines run on the same XCD (same L2).
+// k_ir_lane any shape: one lane per system, FP32 factors in the caller's
+//           scratch buffer (fallback for lengths the partitioned kernels do
+//           not cover).
+#include "ir_core.hpp"
+#include "rows_io.hpp"
+#include "rotated_store.hpp"
+#include "pentad_internal.hpp"
+
+namespace pentad {
+
+// Largest step count over the launch.  Every wavefront reaching one address
+// with an atomic serialises (hundreds of thousands per launch), so the
+// current value is read first and the atomic issued only when it grows --
+// after the first few wavefronts, almost never.
+__device__ __forceinline__ void note_steps(int *steps_out, int k) {
+  if (steps_out == nullptr) return;
+  const int cur = __hip_atomic_load(steps_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (k > cur) atomicMax(steps_out, k);
+}
+
+// ---------------------------------------------------------------------------
+// ROT as in k_part_x: false = in place on x ([sys][row]); true = RHS from b,
+// solution to out in [row][sys] layout, staged through LDS.
+// The coefficients live in LDS ([5][M][64] per wavefront) for the whole
+// refinement; the RHS is re-read from memory for each residual (the RHS is
+// not written before the final store, in place or not).
+template <int M, int R, bool ROT>
+__global__ void __launch_bounds__(256)
+    k_ir_x(const float *__restrict__ ds, const float *__restrict__ dl,
+           const float *__restrict__ d, const float *__restrict__ du,
+           const float *__restrict__ dw, const double *bin,
+           double *out, int64_t n_sys, int64_t len, int max_steps,
+           double tol, int *__restrict__ steps_out, XMap map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int L = 1 << R;
+  const int64_t blk = ROT ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t g = blk * blockDim.x + threadIdx.x;
+  const int64_t sys = g >> R;
+  const int p = (int)(g & (L - 1));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool valid = sys < n_sys;
+  const int64_t sc = valid ? sys : n_sys - 1;
+  const int64_t slo = map.lo(sc), shi = map.hi(sc);
+  const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+  const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
+  float *alds = reinterpret_cast<float *>(smem_raw) + wave * (5 * M * 64) + lane;
+  {
+    float t[M];
+    const float *src[5] = {ds, dl, d, du, dw};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      RowsIO<float, M>::load(src[k] + offc, t);
+#pragma unroll
+      for (int i = 0; i < M; ++i) alds[(k * M + i) * 64] = t[i];
+    }
+  }
+  double xs[M];
+  WaveEx ex{lane, p, L};
+  const LdsCoef<M> A{alds};
+  const MemRhs b{bin + off, 1};
+  const int k = ir_solve<M, R>(A, b, xs, p, L, ex, max_steps, tol);
+  if constexpr (ROT) {
+    stage_rotated_store<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, L,
+                                   len, out, n_sys, blk, map);
+  } else {
+    if (valid) RowsIO<double, M>::store(out + off, xs);
+  }
+  if (lane == 0) note_steps(steps_out, k);
+}
+
+template <int M, int R, int MAXT>
+__global__ void __launch_bounds__(MAXT)
+    k_ir_yz(const float *__restrict__ ds, const float *__restrict__ dl,
+            const float *__restrict__ d, const float *__restrict__ du,
+            const float *__restrict__ dw, double *__restrict__ x, Lines ln,
+            int S, int P, int max_steps, double tol, int *__restrict__ steps_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int t = threadIdx.x;
+  const int sl = t % S;
+  const int p = t / S;
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t sys = tile * S + sl;
+  const bool valid = sys < ln.n_sys;
+  const int64_t st = ln.stride;
+  const int64_t b0 = ln.base(valid ? sys : ln.n_sys - 1) + (int64_t)p * M * st;
+  float s[M], l[M], dd[M], u[M], w[M];
+  double b[M], xs[M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int64_t e = b0 + i * st;
+    s[i] = ds[e];
+    l[i] = dl[e];
+    dd[i] = d[e];
+    u[i] = du[e];
+    w[i] = dw[e];
+    b[i] = x[e];
+  }
+  LdsEx ex{smem_raw, sl, p, S, P};
+  const RegCoef<M> A{s, l, dd, u, w};
+  const RegRhs<M> bv{b};
+  const int k = ir_solve<M, R>(A, bv, xs, p, P, ex, max_steps, tol);
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[b0 + i * st] = xs[i];
+  }
+  if (t == 0) note_steps(steps_out, k);
+}
+
+// ---------------------------------------------------------------------------
+// fallback: one lane per system, factors in scratch [5][len][n_sys] (FP32)
+// plus a copy of b [len][n_sys] (FP64).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_ir_lane(const float *__restrict__ ds, const float *__restrict__ dl,
+              const float *__restrict__ d, const float *__restrict__ du,
+              const float *__restrict__ dw, double *__restrict__ x, Lines ln,
+              float *__restrict__ fs, double *__restrict__ bs, int max_steps,
+              double tol, int *__restrict__ steps_out) {
+  const int64_t sys = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sys >= ln.n_sys) return;
+  const int64_t n = ln.len, st = ln.stride, ns = ln.n_sys;
+  const int64_t b0 = ln.base(sys);
+  float *du2 = fs + sys, *dw2 = du2 + n * ns, *ddl = dw2 + n * ns,
+        *ivs = ddl + n * ns, *z = ivs + n * ns;
+  double *bb = bs + sys;
+  // factor + first solve (FP32)
+  double xmax = 0.0;
+  {
+    float u1 = 0, w1 = 0, z1 = 0, u0 = 0, w0 = 0, z0 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t e = b0 + i * st;
+      const double bi = x[e];
+      bb[i * ns] = bi;
+      const float s = (i >= 2) ? ds[e] : 0.0f;
+      const float lv = (i >= 1) ? dl[e] : 0.0f;
+      const float dl_ = lv - s * u1;
+      const float iv = recip(d[e] - s * w1 - dl_ * u0);
+      // couplings past the end of the line are out of range: never read
+      const float un = (i + 1 < n) ? (du[e] - dl_ * w0) * iv : 0.0f;
+      const float wn = (i + 2 < n) ? dw[e] * iv : 0.0f;
+      const float zn = ((float)bi - s * z1 - dl_ * z0) * iv;
+      du2[i * ns] = un;
+      dw2[i * ns] = wn;
+      ddl[i * ns] = dl_;
+      ivs[i * ns] = iv;
+      z[i * ns] = zn;
+      u1 = u0; w1 = w0; z1 = z0;
+      u0 = un; w0 = wn; z0 = zn;
+    }
+    float xa = 0, xb = 0;
+    for (int64_t i = n - 1; i >= 0; --i) {
+      const float xi = z[i * ns] - du2[i * ns] * xa - dw2[i * ns] * xb;
+      x[b0 + i * st] = (double)xi;
+      xmax = fmax(xmax, fabs((double)xi));
+      xb = xa;
+      xa = xi;
+    }
+  }
+  int k = 0;
+  double dprev = xmax; // dx_0 := x_0
+  while (k < max_steps) {
+    // residual + forward substitution with the stored factors
+    float z1 = 0, z0 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t e = b0 + i * st;
+      double acc = (double)d[e] * x[e];
+      if (i >= 2) acc += (double)ds[e] * x[e - 2 * st];
+      if (i >= 1) acc += (double)dl[e] * x[e - st];
+      if (i + 1 < n) acc += (double)du[e] * x[e + st];
+      if (i + 2 < n) acc += (double)dw[e] * x[e + 2 * st];
+      const float r = (float)(bb[i * ns] - acc);
+      const float s = (i >= 2) ? ds[e] : 0.0f;
+      const float zn = (r - s * z1 - ddl[i * ns] * z0) * ivs[i * ns];
+      z[i * ns] = zn;
+      z1 = z0;
+      z0 = zn;
+    }
+    float xa = 0, xb = 0;
+    double dxm = 0.0;
+    for (int64_t i = n - 1; i >= 0; --i) {
+      const float xi = z[i * ns] - du2[i * ns] * xa - dw2[i * ns] * xb;
+      x[b0 + i * st] += (double)xi;
+      dxm = fmax(dxm, fabs((double)xi));
+      xb = xa;
+      xa = xi;
+    }
+    ++k;
+    // estimated forward error of the new iterate: |dx_k|^2 / |dx_{k-1}|
+    const bool conv = (dxm == 0.0) || (dxm * dxm <= tol * xmax * dprev);
+    dprev = dxm;
+    if (conv) break;
+  }
+  note_steps(steps_out, k);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+int ceil_log2(int64_t v) {
+  int r = 0;
+  while (((int64_t)1 << r) < v) ++r;
+  return r;
+}
+
+template <int M, int R>
+hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot,
+                   int64_t n_sys, int64_t len, const XMap &map, int max_steps,
+                   double tol, int *steps, hipStream_t st) {
+  const int block = 256;
+  const int64_t threads = n_sys << R;
+  const int64_t grid = (threads + block - 1) / block;
+  const size_t smem_a = (size_t)(block / 64) * 5 * M * 64 * sizeof(float);
+  if (rot) {
+    const size_t tile = rotated_tile_bytes<double>(block, M, len);
+    hipLaunchKernelGGL((k_ir_x<M, R, true>), dim3((unsigned)grid), dim3(block),
+                       smem_a > tile ? smem_a : tile, st, a.ds, a.dl, a.d, a.du, a.dw,
+                       b, out, n_sys, len, max_steps, tol, steps, map);
+  } else {
+    hipLaunchKernelGGL((k_ir_x<M, R, false>), dim3((unsigned)grid), dim3(block), smem_a,
+                       st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps,
+                       tol, steps, map);
+  }
+  return hipGetLastError();
+}
+
+// IR keeps ~ (5 FP32 + 2 FP64 + 6 FP32 factors) per row plus 12 floats per
+// PCR round in registers: workgroups of up to 512 threads (2 waves/SIMD).
+constexpr int IR_YZ_MAXT = 512;
+
+template <int M, int R>
+hipError_t go_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int S,
+                    int max_steps, double tol, int *steps, hipStream_t st) {
+  const int P = (int)(ln.len / M);
+  const int block = S * P;
+  if (block > IR_YZ_MAXT || block % 64 != 0) return hipErrorInvalidConfiguration;
+  const size_t smem = (size_t)8 * S * P * sizeof(float) > (size_t)4 * S * P * sizeof(double)
+                          ? (size_t)8 * S * P * sizeof(float)
+                          : (size_t)4 * S * P * sizeof(double);
+  const int64_t grid = (ln.n_sys + S - 1) / S;
+  hipLaunchKernelGGL((k_ir_yz<M, R, IR_YZ_MAXT>), dim3((unsigned)grid), dim3(block),
+                     smem, st, a.ds, a.dl, a.d, a.du, a.dw, x, ln, S, P, max_steps,
+                     tol, steps);
+  return hipGetLastError();
+}
+
+} // namespace
+
+bool ir_x_supported(int rows, int64_t len) {
+  if (rows != 4 && rows != 6 && rows != 8 && rows != 10) return false;
+  if (len % rows) return false;
+  const int64_t L = len / rows;
+  return L >= 2 && L <= 64 && (L & (L - 1)) == 0;
+}
+
+bool ir_yz_supported(int rows, int sys_per_wg, int64_t len) {
+  if (rows != 4 && rows != 6 && rows != 8 && rows != 10) return false;
+  if (len % rows) return false;
+  const int64_t P = len / rows;
+  if (P < 1 || P > 128) return false;
+  const int64_t block = P * sys_per_wg;
+  return block <= IR_YZ_MAXT && block % 64 == 0;
+}
+
+hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
+                           bool rot, int64_t n_sys, int64_t len, const XMap &map,
+                           int rows, int max_steps, double tol, int *steps,
+                           hipStream_t st) {
+  if (!ir_x_supported(rows, len)) return hipErrorInvalidValue;
+  const int R = ceil_log2(len / rows);
+#define IRX(MM, RR) \
+  if (rows == MM && R == RR) \
+    return go_ir_x<MM, RR>(a, b, out, rot, n_sys, len, map, max_steps, tol, steps, st);
+  IRX(4, 1) IRX(4, 2) IRX(4, 3) IRX(4, 4) IRX(4, 5) IRX(4, 6)
+  IRX(6, 1) IRX(6, 2) IRX(6, 3) IRX(6, 4) IRX(6, 5) IRX(6, 6)
+  IRX(8, 1) IRX(8, 2) IRX(8, 3) IRX(8, 4) IRX(8, 5) IRX(8, 6)
+  IRX(10, 1) IRX(10, 2) IRX(10, 3) IRX(10, 4) IRX(10, 5) IRX(10, 6)
+#undef IRX
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ir_x(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                       int max_steps, double tol, int *steps, hipStream_t st) {
+  return launch_ir_x_map(a, x, x, false, ln.n_sys, ln.len, xmap_inplace(ln), rows,
+                         max_steps, tol, steps, st);
+}
+
+hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                        int sys_per_wg, int max_steps, double tol, int *steps,
+                        hipStream_t st) {
+  if (!ir_yz_supported(rows, sys_per_wg, ln.len)) return hipErrorInvalidValue;
+  const int R = ceil_log2(ln.len / rows);
+#define IRYZ(MM, RR) \
+  if (rows == MM && R == RR) return go_ir_yz<MM, RR>(a, x, ln, sys_per_wg, max_steps, tol, steps, st);
+  IRYZ(4, 0) IRYZ(4, 1) IRYZ(4, 2) IRYZ(4, 3) IRYZ(4, 4) IRYZ(4, 5) IRYZ(4, 6) IRYZ(4, 7)
+  IRYZ(6, 0) IRYZ(6, 1) IRYZ(6, 2) IRYZ(6, 3) IRYZ(6, 4) IRYZ(6, 5) IRYZ(6, 6) IRYZ(6, 7)
+  IRYZ(8, 0) IRYZ(8, 1) IRYZ(8, 2) IRYZ(8, 3) IRYZ(8, 4) IRYZ(8, 5) IRYZ(8, 6) IRYZ(8, 7)
+  IRYZ(10, 0) IRYZ(10, 1) IRYZ(10, 2) IRYZ(10, 3) IRYZ(10, 4) IRYZ(10, 5) IRYZ(10, 6) IRYZ(10, 7)
+#undef IRYZ
+  return hipErrorInvalidValue;
+}
+
+size_t ir_lane_scratch_bytes(const Lines &ln) {
+  const size_t n = (size_t)ln.n_sys * (size_t)ln.len;
+  return n * (5 * sizeof(float) + sizeof(double));
+}
+
+hipError_t launch_ir_lane(const Diags<float> &a, double *x, const Lines &ln,
+                          void *scratch, int max_steps, double tol, int *steps,
+                          hipStream_t st) {
+  const size_t n = (size_t)ln.n_sys * (size_t)ln.len;
+  double *bs = static_cast<double *>(scratch);
+  float *fs = reinterpret_cast<float *>(bs + n);
+  const int block = 256;
+  const int64_t grid = (ln.n_sys + block - 1) / block;
+  hipLaunchKernelGGL(k_ir_lane, dim3((unsigned)grid), dim3(block), 0, st, a.ds, a.dl,
+                     a.d, a.du, a.dw, x, ln, fs, bs, max_steps, tol, steps);
+  return hipGetLastError();
+}
+
+} // namespace pentad

@@ -1,0 +1,132 @@
+// kernels_layout.hip -- layout transforms used by the ADI driver.
+//
+// rotate: view the input as a [rows][cols] matrix (cols fastest) and write
+// its transpose [cols][rows], converting the element type on the way
+// (double -> double, double -> float, float -> float, float -> double).
+// With rows = number of lines and cols = line length this turns the
+// [system][row] layout of one ADI direction into the [row][system] layout in
+// which the next direction's lines are contiguous.  64x64 tiles staged
+// through LDS (padded against bank conflicts); reads and writes are 256-byte
+// (FP32) / 512-byte (FP64) coalesced runs.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pentad {
+
+constexpr int TILE = 64;
+constexpr int TROWS = 16; // threads per tile column: 64 x 16 threads
+
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(TILE * TROWS)
+    k_rotate(const Ti *__restrict__ in, To *__restrict__ out, int64_t rows,
+             int64_t cols) {
+  __shared__ To tile[TILE][TILE + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * TILE, c0 = (int64_t)blockIdx.y * TILE;
+  const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+#pragma unroll
+  for (int k = 0; k < TILE; k += TROWS) {
+    const int64_t r = r0 + ty + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = (To)in[r * cols + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < TILE; k += TROWS) {
+    const int64_t c = c0 + ty + k, r = r0 + tx;
+    if (r < rows && c < cols) out[c * rows + r] = tile[tx][ty + k];
+  }
+}
+
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(256)
+    k_convert(const Ti *__restrict__ in, To *__restrict__ out, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (To)in[i];
+}
+
+// in_bytes / out_bytes: 4 or 8.  rows * cols elements.
+hipError_t launch_rotate(const void *in, int in_bytes, void *out, int out_bytes,
+                         int64_t rows, int64_t cols, hipStream_t st) {
+  const dim3 grid((unsigned)((rows + TILE - 1) / TILE), (unsigned)((cols + TILE - 1) / TILE));
+  const dim3 block(TILE * TROWS);
+  if (grid.y > 65535u) return hipErrorInvalidValue; // lines longer than 4M rows
+  if (in_bytes == 8 && out_bytes == 8)
+    hipLaunchKernelGGL((k_rotate<double, double>), grid, block, 0, st,
+                       (const double *)in, (double *)out, rows, cols);
+  else if (in_bytes == 8 && out_bytes == 4)
+    hipLaunchKernelGGL((k_rotate<double, float>), grid, block, 0, st,
+                       (const double *)in, (float *)out, rows, cols);
+  else if (in_bytes == 4 && out_bytes == 4)
+    hipLaunchKernelGGL((k_rotate<float, float>), grid, block, 0, st,
+                       (const float *)in, (float *)out, rows, cols);
+  else if (in_bytes == 4 && out_bytes == 8)
+    hipLaunchKernelGGL((k_rotate<float, double>), grid, block, 0, st,
+                       (const float *)in, (double *)out, rows, cols);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(const void *in, int in_bytes, void *out, int out_bytes,
+                          int64_t n, hipStream_t st) {
+  const int block = 256;
+  int64_t grid = (n + block - 1) / block;
+  if (grid > 256 * 32) grid = 256 * 32;
+  if (in_bytes == 8 && out_bytes == 4)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)grid), dim3(block), 0,
+                       st, (const double *)in, (float *)out, n);
+  else if (in_bytes == 4 && out_bytes == 8)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)grid), dim3(block), 0,
+                       st, (const float *)in, (double *)out, n);
+  else if (in_bytes == out_bytes)
+    return hipMemcpyAsync(out, in, (size_t)n * in_bytes, hipMemcpyDeviceToDevice, st);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// out[i*os0 + j*os1 + k*os2] = in[i + d0*(j + d1*k)]: any axis permutation
+// of a 3-D array (reads coalesced, writes scattered; used once at set-up).
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(256)
+    k_permute3(const Ti *__restrict__ in, To *__restrict__ out, int64_t d0, int64_t d1,
+               int64_t n, int64_t os0, int64_t os1, int64_t os2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    const int64_t i = e % d0, jk = e / d0;
+    const int64_t j = jk % d1, k = jk / d1;
+    out[i * os0 + j * os1 + k * os2] = (To)in[e];
+  }
+}
+
+hipError_t launch_permute3(const void *in, int in_bytes, void *out, int out_bytes,
+                           const int64_t dims[3], const int64_t ostride[3],
+                           hipStream_t st) {
+  const int64_t n = dims[0] * dims[1] * dims[2];
+  const int block = 256;
+  int64_t grid = (n + block - 1) / block;
+  if (grid > 256 * 64) grid = 256 * 64;
+#define PERM(TI, TO)                                                                   \
+  hipLaunchKernelGGL((k_permute3<TI, TO>), dim3((unsigned)grid), dim3(block), 0, st,  \
+                     (const TI *)in, (TO *)out, dims[0], dims[1], n, ostride[0],      \
+                     ostride[1], ostride[2])
+  if (in_bytes == 8 && out_bytes == 8) PERM(double, double);
+  else if (in_bytes == 8 && out_bytes == 4) PERM(double, float);
+  else if (in_bytes == 4 && out_bytes == 4) PERM(float, float);
+  else if (in_bytes == 4 && out_bytes == 8) PERM(float, double);
+  else return hipErrorInvalidValue;
+#undef PERM
+  return hipGetLastError();
+}
+
+__global__ void k_fill_int(int *p, int v) { *p = v; }
+
+// Writes one int with a kernel (usable inside captured graphs, where a
+// memset node is avoided).
+hipError_t launch_fill_int(int *p, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, st, p, v);
+  return hipGetLastError();
+}
+
+} // namespace pentad

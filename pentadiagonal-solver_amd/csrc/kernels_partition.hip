@@ -12,130 +12,56 @@
 // Both keep every row in registers between load and store: HBM traffic is
 // the compulsory 5 coefficient reads + 1 RHS read + 1 solution write per row.
 #include "partition_core.hpp"
+#include "partition_exchange.hpp"
+#include "rows_io.hpp"
+#include "rotated_store.hpp"
 #include "pentad_internal.hpp"
 
 namespace pentad {
 
 // ---------------------------------------------------------------------------
-// exchange policies
-// ---------------------------------------------------------------------------
-struct WaveEx {
-  int lane; // lane id in the wavefront
-  int p;    // chunk index = lane position inside the system's lane group
-  int P;    // lanes per system
-  template <int K, typename T>
-  __device__ __forceinline__ void swap(const T *send, T *from_prev,
-                                       T *from_next, int h) const {
-    const bool hp = (p - h) >= 0, hn = (p + h) < P;
-    const int sp = hp ? lane - h : lane;
-    const int sn = hn ? lane + h : lane;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const T vp = __shfl(send[k], sp, 64);
-      const T vn = __shfl(send[k], sn, 64);
-      from_prev[k] = hp ? vp : T(0);
-      from_next[k] = hn ? vn : T(0);
-    }
-  }
-};
-
-template <typename T> struct LdsEx {
-  T *lds;   // [K][P][S]
-  int s, p; // system, chunk of this thread
-  int S, P;
-  template <int K, typename U>
-  __device__ __forceinline__ void swap(const U *send, U *from_prev,
-                                       U *from_next, int h) const {
-    const bool hp = (p - h) >= 0, hn = (p + h) < P;
-    const int PS = P * S;
-    __syncthreads(); // earlier readers are done with the buffer
-#pragma unroll
-    for (int k = 0; k < K; ++k) lds[k * PS + p * S + s] = send[k];
-    __syncthreads();
-    const int op = (hp ? p - h : p) * S + s;
-    const int on = (hn ? p + h : p) * S + s;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const U vp = lds[k * PS + op];
-      const U vn = lds[k * PS + on];
-      from_prev[k] = hp ? vp : U(0);
-      from_next[k] = hn ? vn : U(0);
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------
-// vector load/store of M consecutive elements (16-byte aligned)
-// ---------------------------------------------------------------------------
-template <typename T, int M>
-__device__ __forceinline__ void load_rows(const T *__restrict__ src, T (&v)[M]) {
-  constexpr int PER = 16 / sizeof(T);
-  static_assert(M % PER == 0, "row count must fill 16-byte vectors");
-  using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
-  const V *q = reinterpret_cast<const V *>(src);
-#pragma unroll
-  for (int k = 0; k < M / PER; ++k) {
-    const V t = q[k];
-    if constexpr (sizeof(T) == 8) {
-      v[2 * k] = t.x;
-      v[2 * k + 1] = t.y;
-    } else {
-      v[4 * k] = t.x;
-      v[4 * k + 1] = t.y;
-      v[4 * k + 2] = t.z;
-      v[4 * k + 3] = t.w;
-    }
-  }
-}
-
-template <typename T, int M>
-__device__ __forceinline__ void store_rows(T *__restrict__ dst, const T (&v)[M]) {
-  constexpr int PER = 16 / sizeof(T);
-  using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
-  V *q = reinterpret_cast<V *>(dst);
-#pragma unroll
-  for (int k = 0; k < M / PER; ++k) {
-    V t;
-    if constexpr (sizeof(T) == 8) {
-      t.x = v[2 * k];
-      t.y = v[2 * k + 1];
-    } else {
-      t.x = v[4 * k];
-      t.y = v[4 * k + 1];
-      t.z = v[4 * k + 2];
-      t.w = v[4 * k + 3];
-    }
-    q[k] = t;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // x direction
 // ---------------------------------------------------------------------------
-template <typename T, int M>
+// Addressing through XMap (lines contiguous, see pentad_internal.hpp).
+// ROT = false: the solution overwrites the RHS (out == b).
+// ROT = true : the solution goes to out at the mapped positions, staged
+//              through LDS so that each store instruction writes runs.
+//              Workgroups are dealt to XCDs so that workgroups with adjacent
+//              systems (whose runs share 128-byte lines) use one L2.
+template <typename T, int M, bool ROT>
 __global__ void __launch_bounds__(256)
     k_part_x(const T *__restrict__ ds, const T *__restrict__ dl,
              const T *__restrict__ d, const T *__restrict__ du,
-             const T *__restrict__ dw, T *__restrict__ x, int64_t n_sys,
-             int64_t len, int lshift /* log2 lanes per system */) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+             const T *__restrict__ dw, const T *b, T *out, int64_t n_sys,
+             int64_t len, int lshift /* log2 lanes per system */, XMap map) {
+  const int64_t blk = ROT ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t g = blk * blockDim.x + threadIdx.x;
   const int L = 1 << lshift;
   const int64_t sys = g >> lshift;
   const int p = (int)(g & (L - 1));
   const bool valid = sys < n_sys;
-  const int64_t off = (valid ? sys : n_sys - 1) * len + (int64_t)p * M;
+  const int64_t sc = valid ? sys : n_sys - 1;
+  const int64_t slo = map.lo(sc), shi = map.hi(sc);
+  const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+  const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
 
   T s[M], l[M], dd[M], u[M], w[M], f[M];
-  load_rows<T, M>(ds + off, s);
-  load_rows<T, M>(dl + off, l);
-  load_rows<T, M>(d + off, dd);
-  load_rows<T, M>(du + off, u);
-  load_rows<T, M>(dw + off, w);
-  load_rows<T, M>(x + off, f);
+  RowsIO<T, M>::load(ds + offc, s);
+  RowsIO<T, M>::load(dl + offc, l);
+  RowsIO<T, M>::load(d + offc, dd);
+  RowsIO<T, M>::load(du + offc, u);
+  RowsIO<T, M>::load(dw + offc, w);
+  RowsIO<T, M>::load(b + off, f);
 
   WaveEx ex{(int)(threadIdx.x & 63), p, L};
   part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
-  if (valid) store_rows<T, M>(x + off, f);
+  if constexpr (ROT) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    stage_rotated_store<T, M>(reinterpret_cast<T *>(smem_raw), f, p, L, len, out,
+                              n_sys, blk, map);
+  } else {
+    if (valid) RowsIO<T, M>::store(out + off, f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -148,11 +74,11 @@ __global__ void __launch_bounds__(MAXT)
               const T *__restrict__ dw, T *__restrict__ x, Lines ln, int S,
               int P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *lds = reinterpret_cast<T *>(smem_raw);
   const int t = threadIdx.x;
   const int sl = t % S;
   const int p = t / S;
-  const int64_t sys = (int64_t)blockIdx.x * S + sl;
+  // adjacent systems share 128-byte lines: keep neighbouring tiles on one XCD
+  const int64_t sys = xcd_tile(blockIdx.x, gridDim.x) * S + sl;
   const bool valid = sys < ln.n_sys;
   const int64_t st = ln.stride;
   const int64_t b = ln.base(valid ? sys : ln.n_sys - 1) + (int64_t)p * M * st;
@@ -169,7 +95,7 @@ __global__ void __launch_bounds__(MAXT)
     f[i] = x[e];
   }
 
-  LdsEx<T> ex{lds, sl, p, S, P};
+  LdsEx ex{smem_raw, sl, p, S, P};
   part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
   if (valid) {
 #pragma unroll
@@ -183,16 +109,20 @@ __global__ void __launch_bounds__(MAXT)
 namespace {
 
 template <typename T, int M>
-hipError_t go_x(const Diags<T> &a, T *x, const Lines &ln, int lanes,
-                hipStream_t st) {
+hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
+                int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
   const int block = 256;
-  const int64_t threads = ln.n_sys * lanes;
+  const int64_t threads = n_sys * lanes;
   const int64_t grid = (threads + block - 1) / block;
-  hipLaunchKernelGGL((k_part_x<T, M>), dim3((unsigned)grid), dim3(block), 0,
-                     st, a.ds, a.dl, a.d, a.du, a.dw, x, ln.n_sys, ln.len,
-                     lshift);
+  if (rot)
+    hipLaunchKernelGGL((k_part_x<T, M, true>), dim3((unsigned)grid), dim3(block),
+                       rotated_tile_bytes<T>(block, M, len), st, a.ds, a.dl, a.d,
+                       a.du, a.dw, b, out, n_sys, len, lshift, map);
+  else
+    hipLaunchKernelGGL((k_part_x<T, M, false>), dim3((unsigned)grid), dim3(block), 0,
+                       st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
   return hipGetLastError();
 }
 
@@ -233,33 +163,32 @@ hipError_t go_yz(const Diags<T> &a, T *x, const Lines &ln, int S,
 
 bool part_x_supported(int elem_bytes, int rows, int lanes) {
   if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0) return false;
-  if (elem_bytes == 8) return rows == 4 || rows == 6 || rows == 8 || rows == 10 || rows == 12 || rows == 16;
-  return rows == 4 || rows == 8 || rows == 12 || rows == 16;
+  return rows == 4 || rows == 6 || rows == 8 || rows == 10 || rows == 12 ||
+         rows == 16;
+}
+
+template <typename T>
+hipError_t launch_part_x_map(const Diags<T> &a, const T *b, T *out, bool rot,
+                             int64_t n_sys, int64_t len, const XMap &map, int rows,
+                             int lanes, hipStream_t st) {
+  if (!part_x_supported(sizeof(T), rows, lanes) || rows * lanes != len)
+    return hipErrorInvalidValue;
+  switch (rows) {
+  case 4: return go_x<T, 4>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 6: return go_x<T, 6>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 8: return go_x<T, 8>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 10: return go_x<T, 10>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 12: return go_x<T, 12>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 16: return go_x<T, 16>(a, b, out, rot, n_sys, len, map, lanes, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <typename T>
 hipError_t launch_part_x(const Diags<T> &a, T *x, const Lines &ln, int rows,
                          int lanes, hipStream_t st) {
-  if (!part_x_supported(sizeof(T), rows, lanes) || rows * lanes != ln.len)
-    return hipErrorInvalidValue;
-  if constexpr (sizeof(T) == 8) {
-    switch (rows) {
-    case 4: return go_x<T, 4>(a, x, ln, lanes, st);
-    case 6: return go_x<T, 6>(a, x, ln, lanes, st);
-    case 8: return go_x<T, 8>(a, x, ln, lanes, st);
-    case 10: return go_x<T, 10>(a, x, ln, lanes, st);
-    case 12: return go_x<T, 12>(a, x, ln, lanes, st);
-    case 16: return go_x<T, 16>(a, x, ln, lanes, st);
-    }
-  } else {
-    switch (rows) {
-    case 4: return go_x<T, 4>(a, x, ln, lanes, st);
-    case 8: return go_x<T, 8>(a, x, ln, lanes, st);
-    case 12: return go_x<T, 12>(a, x, ln, lanes, st);
-    case 16: return go_x<T, 16>(a, x, ln, lanes, st);
-    }
-  }
-  return hipErrorInvalidValue;
+  return launch_part_x_map<T>(a, x, x, false, ln.n_sys, ln.len, xmap_inplace(ln), rows,
+                              lanes, st);
 }
 
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len) {
@@ -291,6 +220,13 @@ template hipError_t launch_part_x<float>(const Diags<float> &, float *,
 template hipError_t launch_part_x<double>(const Diags<double> &, double *,
                                           const Lines &, int, int,
                                           hipStream_t);
+template hipError_t launch_part_x_map<float>(const Diags<float> &, const float *,
+                                             float *, bool, int64_t, int64_t,
+                                             const XMap &, int, int, hipStream_t);
+template hipError_t launch_part_x_map<double>(const Diags<double> &,
+                                              const double *, double *, bool,
+                                              int64_t, int64_t, const XMap &, int,
+                                              int, hipStream_t);
 template hipError_t launch_part_yz<float>(const Diags<float> &, float *,
                                           const Lines &, int, int,
                                           hipStream_t);

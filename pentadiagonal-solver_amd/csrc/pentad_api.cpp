@@ -202,7 +202,70 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
   return pl;
 }
 
+// FP32 + refinement: partitioned kernels when the length splits into
+// supported chunks, else the one-lane-per-system kernel with scratch.
+Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
+  Plan pl;
+  pl.kind = Plan::IrLane;
+  const std::string want = requested_algo(c, ln.dir);
+  if (want == "naive") return pl;
+  int frows = 0, fwidth = 0;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    frows = c->force_rows[ln.dir];
+    fwidth = c->force_width[ln.dir];
+  }
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->tune.find(TuneKey{TUNE_IR, ln.dir, ln.len});
+    if (frows <= 0 && it != c->tune.end()) {
+      frows = it->second.rows;
+      fwidth = ln.dir == 0 ? it->second.lanes : it->second.sys_per_wg;
+    }
+  }
+  if (ln.dir == 0) {
+    bool al = true;
+    for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
+    if (!al) return pl;
+    if (frows > 0 && ir_x_supported(frows, ln.len)) {
+      pl.kind = Plan::IrX;
+      pl.rows = frows;
+      pl.lanes = (int)(ln.len / frows);
+      return pl;
+    }
+    static const int pref[] = {8, 6, 4, 10};
+    for (int r : pref) {
+      if (ir_x_supported(r, ln.len)) {
+        pl.kind = Plan::IrX;
+        pl.rows = r;
+        pl.lanes = (int)(ln.len / r);
+        return pl;
+      }
+    }
+    return pl;
+  }
+  if (frows > 0 && ir_yz_supported(frows, fwidth, ln.len)) {
+    pl.kind = Plan::IrYZ;
+    pl.rows = frows;
+    pl.sys_per_wg = fwidth;
+    return pl;
+  }
+  static const int pref[] = {8, 6, 4, 10};
+  for (int r : pref) {
+    for (int S = 64; S >= 4; S >>= 1) {
+      if (ir_yz_supported(r, S, ln.len)) {
+        pl.kind = Plan::IrYZ;
+        pl.rows = r;
+        pl.sys_per_wg = S;
+        return pl;
+      }
+    }
+  }
+  return pl;
+}
+
 size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes) {
+  if (pl.kind == Plan::IrLane) return ir_lane_scratch_bytes(ln);
   if (pl.kind == Plan::Naive)
     return (size_t)3 * (size_t)ln.n_sys * (size_t)ln.len * (size_t)elem_bytes;
   return 0;
@@ -219,8 +282,56 @@ std::string plan_config(const Plan &pl) {
     std::snprintf(buf, sizeof buf, "thomas-pcr rows=%d sys_per_wg=%d", pl.rows,
                   pl.sys_per_wg);
     return buf;
+  case Plan::IrX:
+    std::snprintf(buf, sizeof buf, "ir thomas-pcr rows=%d lanes=%d", pl.rows,
+                  pl.lanes);
+    return buf;
+  case Plan::IrYZ:
+    std::snprintf(buf, sizeof buf, "ir thomas-pcr rows=%d sys_per_wg=%d",
+                  pl.rows, pl.sys_per_wg);
+    return buf;
+  case Plan::IrLane:
+    return "ir naive";
   default:
     return "naive";
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch one plan (no fallback); used by the solve and by the autotuner
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_plan(const Plan &pl, const Diags<T> &a, T *x, const Lines &ln,
+                       void *buffer, hipStream_t st) {
+  switch (pl.kind) {
+  case Plan::PartX: return launch_part_x<T>(a, x, ln, pl.rows, pl.lanes, st);
+  case Plan::PartYZ:
+    return launch_part_yz<T>(a, x, ln, pl.rows, pl.sys_per_wg, st);
+  case Plan::Naive:
+    if (buffer == nullptr) return hipErrorInvalidValue;
+    return launch_thomas_lane<T>(a, x, ln, static_cast<T *>(buffer), st);
+  default: return hipErrorInvalidValue;
+  }
+}
+template hipError_t launch_plan<float>(const Plan &, const Diags<float> &, float *,
+                                       const Lines &, void *, hipStream_t);
+template hipError_t launch_plan<double>(const Plan &, const Diags<double> &,
+                                        double *, const Lines &, void *,
+                                        hipStream_t);
+
+hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
+                          const Lines &ln, void *buffer, int max_steps,
+                          double tol, int *steps, hipStream_t st) {
+  switch (pl.kind) {
+  case Plan::IrX:
+    return launch_ir_x(a, x, ln, pl.rows, max_steps, tol, steps, st);
+  case Plan::IrYZ:
+    return launch_ir_yz(a, x, ln, pl.rows, pl.sys_per_wg, max_steps, tol, steps,
+                        st);
+  case Plan::IrLane:
+    if (buffer == nullptr) return hipErrorInvalidValue;
+    return launch_ir_lane(a, x, ln, buffer, max_steps, tol, steps, st);
+  default: return hipErrorInvalidValue;
   }
 }
 
@@ -247,11 +358,7 @@ pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
   hipStream_t st = c->stream;
   hipError_t err = hipSuccess;
 
-  if (pl.kind == Plan::PartX) {
-    err = launch_part_x<T>(a, x, ln, pl.rows, pl.lanes, st);
-  } else if (pl.kind == Plan::PartYZ) {
-    err = launch_part_yz<T>(a, x, ln, pl.rows, pl.sys_per_wg, st);
-  }
+  if (pl.kind != Plan::Naive) err = launch_plan<T>(pl, a, x, ln, nullptr, st);
   if (pl.kind != Plan::Naive && err != hipSuccess) {
     // an opt-in kernel that declines falls back, as the reference does
     // (src/cuda/singlenode/pentad_cuda.cu:17-37)
@@ -279,6 +386,61 @@ pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
               plan_config(pl));
   return PENTAD_STATUS_SUCCESS;
 }
+
+pentadStatus_t gpsv_batch_ir(Context *c, const float *ds, const float *dl,
+                             const float *d, const float *du, const float *dw,
+                             double *x, const int *dims, int ndim, int solvedim,
+                             int max_steps, double tol, int *steps,
+                             void *buffer) {
+  Lines ln;
+  if (decompose(dims, ndim, solvedim, &ln) != 0 || ln.len < 3 ||
+      max_steps < 0) {
+    std::fprintf(stderr, "[penta] ir: invalid arguments (ndim %d, solvedim %d)\n",
+                 ndim, solvedim);
+    return PENTAD_STATUS_INVALID_VALUE;
+  }
+  if (!ds || !dl || !d || !du || !dw || !x) {
+    std::fprintf(stderr, "[penta] ir: null array pointer\n");
+    return PENTAD_STATUS_INVALID_VALUE;
+  }
+  const void *ptrs[6] = {ds, dl, d, du, dw, x};
+  Plan pl = plan_ir(c, ln, ptrs);
+  const Diags<float> a{ds, dl, d, du, dw};
+  hipError_t err = hipSuccess;
+  if (pl.kind != Plan::IrLane)
+    err = launch_plan_ir(pl, a, x, ln, nullptr, max_steps, tol, steps, c->stream);
+  if (pl.kind != Plan::IrLane && err != hipSuccess) {
+    if (debug_launch())
+      std::fprintf(stderr, "[penta] %s declined: %s -- falling back\n",
+                   plan_config(pl).c_str(), hipGetErrorString(err));
+    pl = Plan{};
+    pl.kind = Plan::IrLane;
+  }
+  if (pl.kind == Plan::IrLane) {
+    if (buffer == nullptr) {
+      std::fprintf(stderr,
+                   "[penta] ir solvedim %d needs %zu bytes of scratch; got NULL\n",
+                   solvedim, ir_lane_scratch_bytes(ln));
+      return PENTAD_STATUS_INVALID_VALUE;
+    }
+    err = launch_plan_ir(pl, a, x, ln, buffer, max_steps, tol, steps, c->stream);
+  }
+  if (err != hipSuccess) {
+    std::fprintf(stderr, "[penta] ir solvedim %d launch FAILED: %s -- output is invalid\n",
+                 solvedim, hipGetErrorString(err));
+    return PENTAD_STATUS_EXECUTION_FAILED;
+  }
+  note_kernel(c, ln.dir, "ir", plan_config(pl));
+  return PENTAD_STATUS_SUCCESS;
+}
+
+template pentadStatus_t gpsv_batch<float>(Context *, const float *, const float *,
+                                          const float *, const float *, const float *,
+                                          float *, const int *, int, int, void *);
+template pentadStatus_t gpsv_batch<double>(Context *, const double *, const double *,
+                                           const double *, const double *,
+                                           const double *, double *, const int *, int,
+                                           int, void *);
 
 template <typename T>
 size_t buffer_size(Context *c, const T *ds, const T *dl, const T *d,
@@ -353,6 +515,28 @@ void pentadsolver_S_gpsv_batch(pentadsolver_handle_t h, const float *ds,
 }
 
 const char *pentadsolver_kernel_that_ran(int dir) { return ran_global(dir); }
+
+size_t pentadsolver_DS_gpsv_batch_ir_buffer_size_ext(
+    pentadsolver_handle_t h, const float *ds, const float *dl, const float *d,
+    const float *du, const float *dw, const double *x, const int *t_dims,
+    int t_ndim, int t_solvedim) {
+  Lines ln;
+  if (decompose(t_dims, t_ndim, t_solvedim, &ln) != 0) return 0;
+  const void *ptrs[6] = {ds, dl, d, du, dw, x};
+  return plan_scratch_bytes(plan_ir(ctx_of(h), ln, ptrs), ln, 4);
+}
+
+void pentadsolver_DS_gpsv_batch_ir(pentadsolver_handle_t h, const float *ds,
+                                   const float *dl, const float *d,
+                                   const float *du, const float *dw, double *x,
+                                   const int *t_dims, int t_ndim,
+                                   int t_solvedim, int max_refine, double tol,
+                                   int *d_refine_steps, void *t_buffer) {
+  Context *c = ctx_of(h);
+  set_status(c, gpsv_batch_ir(c, ds, dl, d, du, dw, x, t_dims, t_ndim,
+                              t_solvedim, max_refine, tol, d_refine_steps,
+                              t_buffer));
+}
 
 pentadStatus_t pentadsolver_get_last_status(pentadsolver_handle_t h) {
   return ctx_of(h)->last_status;

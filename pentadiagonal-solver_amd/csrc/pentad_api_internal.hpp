@@ -13,7 +13,7 @@ namespace pentad {
 
 // Which kernel a solve launches and with what shape parameters.
 struct Plan {
-  enum Kind { Naive = 0, PartX = 1, PartYZ = 2 } kind = Naive;
+  enum Kind { Naive = 0, PartX = 1, PartYZ = 2, IrLane = 3, IrX = 4, IrYZ = 5 } kind = Naive;
   int rows = 0;       // rows per lane (partitioned kernels)
   int lanes = 0;      // lanes per system (x)
   int sys_per_wg = 0; // systems per workgroup (y/z)
@@ -30,6 +30,7 @@ struct TuneKey {
   }
 };
 using TuneVal = Plan;
+constexpr int TUNE_IR = 12; // TuneKey::elem_bytes of the FP32+FP64 solve
 
 struct Context {
   std::mutex mu;
@@ -57,7 +58,14 @@ bool debug_launch();
 
 Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
                 const void *const ptrs[6]);
+Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]);
 size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes);
+template <typename T>
+hipError_t launch_plan(const Plan &pl, const Diags<T> &a, T *x, const Lines &ln,
+                       void *buffer, hipStream_t st);
+hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
+                          const Lines &ln, void *buffer, int max_steps,
+                          double tol, int *steps, hipStream_t st);
 std::string plan_config(const Plan &pl);
 
 } // namespace pentad

@@ -61,6 +61,37 @@ template <typename T>
 hipError_t launch_part_x(const Diags<T> &a, T *x, const Lines &ln, int rows,
                          int lanes, hipStream_t st);
 bool part_x_supported(int elem_bytes, int rows, int lanes);
+// Addressing of a batch of contiguous lines for the x-like kernels (used by
+// the ADI driver, whose field and coefficient layouts differ per direction).
+// System s splits as s_lo = s % S1, s_hi = s / S1 (64-bit):
+//   RHS line start          s_lo * in_lo  + s_hi * in_hi
+//   coefficient line start  s_lo * c_lo   + s_hi * c_hi
+//   solution element row r  s_lo * out_lo + r * out_row + s_hi * out_hi
+// (rows of a line are contiguous on input).  A workgroup owns consecutive
+// s, so with out_lo = 1 its solutions form runs along s_lo in the output.
+struct XMap {
+  int64_t S1 = 1;
+  int64_t in_lo = 0, in_hi = 0;
+  int64_t c_lo = 0, c_hi = 0;
+  int64_t out_lo = 0, out_row = 1, out_hi = 0;
+  __host__ __device__ __forceinline__ int64_t lo(int64_t s) const { return s % S1; }
+  __host__ __device__ __forceinline__ int64_t hi(int64_t s) const { return s / S1; }
+};
+// lines [sys][row] contiguous, coefficients alike, in place
+inline XMap xmap_inplace(const Lines &ln) {
+  XMap m;
+  m.S1 = ln.n_sys > 0 ? ln.n_sys : 1;
+  m.in_lo = m.c_lo = m.out_lo = ln.len;
+  m.out_row = 1;
+  return m;
+}
+// The same solves with the mapped addressing; rot = false writes the
+// solution over the RHS (out == b), rot = true writes it at the mapped
+// output positions (out != b), staged through LDS.
+template <typename T>
+hipError_t launch_part_x_map(const Diags<T> &a, const T *b, T *out, bool rot,
+                             int64_t n_sys, int64_t len, const XMap &map, int rows,
+                             int lanes, hipStream_t st);
 
 // Partitioned solve, strided (y/z) directions: a workgroup owns `sys_per_wg`
 // adjacent systems and splits each into len/rows chunks, one lane per
@@ -69,5 +100,24 @@ template <typename T>
 hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
                           int sys_per_wg, hipStream_t st);
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
+
+// FP32 coefficients + FP64 RHS/solution with on-chip iterative refinement.
+// steps (device int, may be null) receives atomicMax of the refinement
+// steps taken.
+bool ir_x_supported(int rows, int64_t len);
+bool ir_yz_supported(int rows, int sys_per_wg, int64_t len);
+hipError_t launch_ir_x(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                       int max_steps, double tol, int *steps, hipStream_t st);
+hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
+                           bool rot, int64_t n_sys, int64_t len, const XMap &map,
+                           int rows, int max_steps, double tol, int *steps,
+                           hipStream_t st);
+hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                        int sys_per_wg, int max_steps, double tol, int *steps,
+                        hipStream_t st);
+size_t ir_lane_scratch_bytes(const Lines &ln);
+hipError_t launch_ir_lane(const Diags<float> &a, double *x, const Lines &ln,
+                          void *scratch, int max_steps, double tol, int *steps,
+                          hipStream_t st);
 
 } // namespace pentad

@@ -1,0 +1,125 @@
+"""FP32 solve + FP64 iterative refinement on the GPU vs the FP64 oracle.
+
+The oracle solves widen(A32) x = b in FP64 (the system the refinement
+converges to).  Bar: max-norm relative forward error <= 1e-12 for both the
+well-conditioned (dominance ratio 2-3) and the weakly dominant (1.05) sets.
+"""
+import numpy as np
+import pytest
+
+import pentadsolver_amd as P
+from oracle import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def gen32(dims, seed, ratio=(2.0, 3.0)):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    ds, dl, du, dw = (rng.uniform(-1, 1, n) for _ in range(4))
+    r = rng.uniform(*ratio, n) if ratio[1] > ratio[0] else np.full(n, ratio[0])
+    d = r * (abs(ds) + abs(dl) + abs(du) + abs(dw))
+    b = rng.uniform(-1, 1, n)
+    a32 = [a.astype(np.float32) for a in (ds, dl, d, du, dw)]
+    return a32, b
+
+
+def run_ir(a32, b, dims, sd, handle=None, max_refine=5, tol=1e-13):
+    t = [torch.from_numpy(a).cuda() for a in a32]
+    x = torch.from_numpy(b.copy()).cuda()
+    steps = torch.zeros(1, dtype=torch.int32, device="cuda")
+    P.solve_ir(*t, x, dims, sd, max_refine=max_refine, tol=tol, steps=steps, handle=handle)
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), int(steps.item())
+
+
+def reference(a32, b, dims, sd):
+    a64 = [a.astype(np.float64) for a in a32]
+    return O.solve(*a64, b.copy(), dims, sd)
+
+
+def relerr(a, ref):
+    return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+
+
+SHAPES = [([512, 16, 8], 0), ([16, 512, 8], 1), ([16, 8, 512], 2), ([256, 32, 16], 0),
+          ([32, 256, 16], 1), ([16, 32, 256], 2), ([384, 12, 10], 0), ([10, 12, 384], 2),
+          ([64, 64, 64], 0), ([64, 64, 64], 1), ([64, 64, 64], 2),
+          ([37, 9, 7], 0), ([9, 37, 7], 1), ([7, 9, 37], 2)]
+
+
+@pytest.mark.parametrize("dims,sd", SHAPES)
+@pytest.mark.parametrize("cond", ["well", "weak"])
+def test_ir_matches_fp64_oracle(dims, sd, cond):
+    ratio = (2.0, 3.0) if cond == "well" else (1.05, 1.05)
+    a32, b = gen32(dims, 3 if cond == "well" else 4, ratio)
+    print(f"ir dims={dims} sd={sd} {cond}", flush=True)
+    x, steps = run_ir(a32, b, dims, sd)
+    err = relerr(x, reference(a32, b, dims, sd))
+    print(f"  err={err:.2e} steps={steps}", flush=True)
+    assert err < TOL
+    assert 1 <= steps <= 5
+
+
+def test_ir_fixed_steps_when_tol_zero():
+    dims = [256, 16, 4]
+    a32, b = gen32(dims, 5)
+    x, steps = run_ir(a32, b, dims, 0, max_refine=3, tol=0.0)
+    assert steps == 3
+    assert relerr(x, reference(a32, b, dims, 0)) < TOL
+
+
+def test_ir_zero_steps_is_plain_fp32():
+    dims = [256, 16, 4]
+    a32, b = gen32(dims, 6)
+    x, steps = run_ir(a32, b, dims, 1, max_refine=0)
+    err = relerr(x, reference(a32, b, dims, 1))
+    assert steps == 0
+    assert 1e-12 < err < 2e-5  # FP32 accuracy only, refinement needed
+
+
+IRX = [(4, 8), (4, 64), (6, 16), (6, 64), (8, 4), (8, 32), (8, 64), (10, 16), (10, 32)]
+
+
+def test_ir_x_all_configs():
+    with P.Handle() as h:
+        for rows, lanes in IRX:
+            n = rows * lanes
+            dims = [n, 21, 3]
+            print(f"irx rows={rows} lanes={lanes}", flush=True)
+            h.set_kernel_params(0, rows, lanes)
+            a32, b = gen32(dims, rows + lanes)
+            x, _ = run_ir(a32, b, dims, 0, handle=h)
+            assert h.kernel_config(0) == f"ir thomas-pcr rows={rows} lanes={lanes}"
+            assert relerr(x, reference(a32, b, dims, 0)) < TOL
+
+
+IRYZ = [(4, 16, 64), (4, 8, 256), (6, 8, 192), (8, 16, 256), (8, 8, 512), (8, 64, 64),
+        (10, 8, 640), (10, 16, 160), (6, 4, 768)]
+
+
+@pytest.mark.parametrize("sd", [1, 2])
+def test_ir_yz_all_configs(sd):
+    with P.Handle() as h:
+        for rows, spw, n in IRYZ:
+            dims = [24, n, 3] if sd == 1 else [24, 3, n]
+            print(f"iryz sd={sd} rows={rows} spw={spw} n={n}", flush=True)
+            h.set_kernel_params(sd, rows, spw)
+            a32, b = gen32(dims, rows * spw)
+            x, _ = run_ir(a32, b, dims, sd, handle=h)
+            assert h.kernel_config(sd) == f"ir thomas-pcr rows={rows} sys_per_wg={spw}"
+            assert relerr(x, reference(a32, b, dims, sd)) < TOL
+
+
+def test_ir_lane_fallback_forced():
+    dims = [256, 20, 6]
+    a32, b = gen32(dims, 9)
+    with P.Handle() as h:
+        for sd in range(3):
+            h.set_algo(sd, "naive")
+            x, steps = run_ir(a32, b, dims, sd, handle=h)
+            assert h.kernel_config(sd) == "ir naive"
+            assert relerr(x, reference(a32, b, dims, sd)) < TOL
