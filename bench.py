@@ -11,9 +11,10 @@ summed over the three directions.
 
 Headline ("value"): FP32 bulk solve + FP64 iterative refinement (coefficients
 stored in FP32, field in FP64).  Beside it: plain FP64 and plain FP32 on the
-same driver, and the in-place drop-in per-direction FP64 solves.  Each mode is
-captured once as a hipGraph and replayed; the timed region brackets exactly
---steps replays with a synchronisation on both sides.  Per-direction kernel
+same driver, and the in-place drop-in per-direction FP64 solves.  Each mode
+captures --steps steps as one hipGraph (after a --warmup-step graph) and the
+timed region brackets exactly one replay of it (--steps steps) with a
+synchronisation on both sides.  Per-direction kernel
 durations come from HIP events recorded on the handle's stream (torch's
 current stream), and the dominant kernel gives the roofline.  After timing,
 one step is re-run from the initial field and compared on the full grid with
@@ -182,6 +183,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-13)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="also launch the 6-read/1-write bandwidth probe once per precision "
+                         "over the same field size (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); lines are "
@@ -212,6 +216,19 @@ def main():
         r = res[mode]
         log(f"bench: {mode} {r['ms_per_step']:.3f} ms/step {r['grows']:.1f} GRows/s dirs "
             f"{['%.3f' % v for v in r['dir_ms']]} kernels {r['kernels']} refine {r['refine_steps_timed_max']}")
+    if args.probe:
+        log("bench: bandwidth probe (6 reads + 1 write per element)")
+        for arrs, out in ((c64 + (b,), torch.empty_like(b)), (c32 + (b32,), torch.empty_like(b32))):
+            P.probe_stream(list(arrs), out, handle=h)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(torch.cuda.current_stream())
+            P.probe_stream(list(arrs), out, handle=h)
+            e1.record(torch.cuda.current_stream())
+            torch.cuda.synchronize()
+            gbs = 7 * out.numel() * out.element_size() / (e0.elapsed_time(e1) * 1e-3) / 1e9
+            log(f"bench: probe {out.dtype}: {gbs:.0f} GB/s")
+            del out
     log("bench: drop-in in-place FP64 direction solves")
     dropin_ms, dropin_k = dropin_direction_ms(P, torch, c64, b, dims, h)
 

@@ -50,7 +50,7 @@ _SIGNATURES = {
     "pentadsolver_S_autotune": (_I, [_P] * 7 + [_IP, _I, _I, _P, _SZ, _I, ctypes.c_char_p, _SZ]),
     "pentadsolver_DS_autotune_ir": (_I, [_P] * 7 + [_IP, _I, _I, _I, ctypes.c_double, _P, _SZ, _I, ctypes.c_char_p, _SZ]),
     "pentadsolver_clear_tuning": (None, [_P]),
-    "pentadsolver_probe_stream": (_I, [ctypes.POINTER(_P), _P, _SZ, _I, _I]),
+    "pentadsolver_probe_stream": (_I, [_P, ctypes.POINTER(_P), _P, _SZ, _I, _I]),
     "pentadsolver_adi_create": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), _I, _I, _I, ctypes.c_double]),
     "pentadsolver_adi_step": (_I, [_P, _P, _I]),
     "pentadsolver_adi_graph_capture": (_I, [_P, _P, _I]),

@@ -80,3 +80,21 @@ def test_host_only_calls_without_gpu():
     assert lib.pentadsolver_kernel_that_ran(5) == b"none"
     lib.pentadsolver_destroy(ctypes.byref(h))
     assert not h.value
+
+
+def declared_param_counts():
+    """name -> number of parameters, from the prototypes in include/pentadsolver.h."""
+    text = open(HEADERS[0]).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(pentadsolver_\w+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_python_bindings_match_header_prototypes():
+    counts = declared_param_counts()
+    for name, (_, argtypes) in P._SIGNATURES.items():
+        assert name in counts, name
+        assert len(argtypes) == counts[name], (name, len(argtypes), counts[name])
