@@ -70,13 +70,25 @@ struct MemRhs { // rows i of this lane's chunk: p[i * stride]
   __device__ __forceinline__ double operator()(int i) const { return p[i * stride]; }
 };
 
+// Zero the couplings of the chunk's rows that reach past the ends of the
+// system (diagonal k: 0 ds, 1 dl, 3 du, 4 dw), so that products with halo
+// values of chunks that do not exist vanish without selects.
+template <int M>
+__device__ __forceinline__ void ir_zero_out_of_range(int k, float (&t)[M], bool first,
+                                                     bool last) {
+  if (k == 0 && first) t[0] = t[1] = 0.0f;
+  if (k == 1 && first) t[0] = 0.0f;
+  if (k == 3 && last) t[M - 1] = 0.0f;
+  if (k == 4 && last) t[M - 2] = t[M - 1] = 0.0f;
+}
+
 // Factorisation of one chunk: du2, 1/pivot, the previous-separator spikes
-// (za, zb), the separator couplings and the block PCR multipliers of each
-// round (MAXR >= ceil(log2 P)).
+// (za, zb) and the block PCR multipliers of each round (MAXR >= ceil(log2 P)).
+// The separator rows' couplings are read from the coefficients when used
+// (the last chunk's out-of-range ones are zero there).
 template <int M, int MAXR> struct IrFactors {
   static constexpr int m = M - 2;
   float du2[m], iv[m], za[m], zb[m];
-  float sa, la, wa, sb, ub, wb;
   M2<float> Bi0;
   M2<float> Ah[MAXR], Ch[MAXR], Bi[MAXR];
 };
@@ -162,33 +174,29 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
       send[c] = XF0[c];
       send[4 + c] = XF1[c];
     }
-    ex.template swap<8>(send, rp, rn, 1);
+    ex.template swapc<8>(send, rp, rn, 1);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       Y0[c] = rn[c];
       Y1[c] = rn[4 + c];
     }
   }
-  F.sa = A.s(m);
-  F.la = A.l(m);
-  F.wa = last ? 0.0f : A.w(m);
-  F.sb = A.s(m + 1);
-  F.ub = last ? 0.0f : A.u(m + 1);
-  F.wb = last ? 0.0f : A.w(m + 1);
+  const float sa = A.s(m), la = A.l(m), wa = last ? 0.0f : A.w(m);
+  const float sb = A.s(m + 1), ub = last ? 0.0f : A.u(m + 1), wb = last ? 0.0f : A.w(m + 1);
   const float da = A.d(m), ua = A.u(m), lb = A.l(m + 1), db = A.d(m + 1);
   M2<float> Am, B, C;
-  Am.a00 = F.sa * XL2[0] + F.la * XL1[0];
-  Am.a01 = F.sa * XL2[1] + F.la * XL1[1];
-  B.a00 = F.sa * XL2[2] + F.la * XL1[2] + da + F.wa * Y0[0];
-  B.a01 = F.sa * XL2[3] + F.la * XL1[3] + ua + F.wa * Y0[1];
-  C.a00 = F.wa * Y0[2];
-  C.a01 = F.wa * Y0[3];
-  Am.a10 = F.sb * XL1[0];
-  Am.a11 = F.sb * XL1[1];
-  B.a10 = F.sb * XL1[2] + lb + F.ub * Y0[0] + F.wb * Y1[0];
-  B.a11 = F.sb * XL1[3] + db + F.ub * Y0[1] + F.wb * Y1[1];
-  C.a10 = F.ub * Y0[2] + F.wb * Y1[2];
-  C.a11 = F.ub * Y0[3] + F.wb * Y1[3];
+  Am.a00 = sa * XL2[0] + la * XL1[0];
+  Am.a01 = sa * XL2[1] + la * XL1[1];
+  B.a00 = sa * XL2[2] + la * XL1[2] + da + wa * Y0[0];
+  B.a01 = sa * XL2[3] + la * XL1[3] + ua + wa * Y0[1];
+  C.a00 = wa * Y0[2];
+  C.a01 = wa * Y0[3];
+  Am.a10 = sb * XL1[0];
+  Am.a11 = sb * XL1[1];
+  B.a10 = sb * XL1[2] + lb + ub * Y0[0] + wb * Y1[0];
+  B.a11 = sb * XL1[3] + db + ub * Y0[1] + wb * Y1[1];
+  C.a10 = ub * Y0[2] + wb * Y1[2];
+  C.a11 = ub * Y0[3] + wb * Y1[3];
   F.Bi0 = inv(B);
   Am = mul(F.Bi0, Am);
   C = mul(F.Bi0, C);
@@ -198,7 +206,7 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
     if (h < P) {
       float send[8] = {Am.a00, Am.a01, Am.a10, Am.a11, C.a00, C.a01, C.a10, C.a11};
       float rp[8], rn[8];
-      ex.template swap<8>(send, rp, rn, h);
+      ex.template swapc<8>(send, rp, rn, h);
       const M2<float> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
       const M2<float> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
       F.Ah[j] = Am;
@@ -239,12 +247,13 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
   float y0, y1;
   {
     float send[2] = {xa, xb}, rp[2], rn[2];
-    ex.template swap<2>(send, rp, rn, 1);
+    ex.template swapc<2>(send, rp, rn, 1);
     y0 = rn[0];
     y1 = rn[1];
   }
-  const float r0 = f[m] - F.sa * xl2 - F.la * xl1 - F.wa * y0;
-  const float r1 = f[m + 1] - F.sb * xl1 - F.ub * y0 - F.wb * y1;
+  // separator couplings (zero past the end of the system)
+  const float r0 = f[m] - A.s(m) * xl2 - A.l(m) * xl1 - A.w(m) * y0;
+  const float r1 = f[m + 1] - A.s(m + 1) * xl1 - A.u(m + 1) * y0 - A.w(m + 1) * y1;
   float q0 = F.Bi0.a00 * r0 + F.Bi0.a01 * r1;
   float q1 = F.Bi0.a10 * r0 + F.Bi0.a11 * r1;
 #pragma unroll
@@ -252,7 +261,7 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
     const int h = 1 << j;
     if (h < P) {
       float send[2] = {q0, q1}, rp[2], rn[2];
-      ex.template swap<2>(send, rp, rn, h);
+      ex.template swapc<2>(send, rp, rn, h);
       const M2<float> &Ah = F.Ah[j], &Ch = F.Ch[j], &Bi = F.Bi[j];
       const float t0 = q0 - (Ah.a00 * rp[0] + Ah.a01 * rp[1]) - (Ch.a00 * rn[0] + Ch.a01 * rn[1]);
       const float t1 = q1 - (Ah.a10 * rp[0] + Ah.a11 * rp[1]) - (Ch.a10 * rn[0] + Ch.a11 * rn[1]);
@@ -263,7 +272,7 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
   float s0, s1;
   {
     float send[2] = {q0, q1}, rp[2], rn[2];
-    ex.template swap<2>(send, rp, rn, 1);
+    ex.template swapc<2>(send, rp, rn, 1);
     s0 = rp[0];
     s1 = rp[1];
   }
@@ -288,27 +297,30 @@ template <int M, int MAXR, typename Cf, typename Bv, typename Ex>
 __device__ __forceinline__ int ir_solve(const Cf &A, const Bv &b, double (&x)[M],
                                         int p, int P, Ex &ex, int max_steps,
                                         double tol) {
-  const bool first = (p == 0), last = (p == P - 1);
   IrFactors<M, MAXR> F;
   ir_factor<M, MAXR>(F, A, p, P, ex);
   float f[M];
 #pragma unroll
   for (int i = 0; i < M; ++i) f[i] = (float)b(i);
   ir_apply<M, MAXR>(F, A, f, p, P, ex);
-  double xloc = 0.0;
+  float xloc = 0.0f;
 #pragma unroll
   for (int i = 0; i < M; ++i) {
     x[i] = (double)f[i];
-    xloc = fmax(xloc, fabs(x[i]));
+    xloc = fmaxf(xloc, fabsf(f[i]));
   }
-  const double xnorm = max_all(ex, xloc, P);
+  // the norms of the stopping rule only steer it: FP32 maxima of FP32 values
+  const double xnorm = (double)ex.max_f32(xloc);
   double dprev = xnorm; // dx_0 := x_0
   int k = 0;
   while (k < max_steps) {
-    // halo: the previous chunk's last two rows, the next chunk's first two
+    // halo: the previous chunk's last two rows, the next chunk's first two.
+    // Where a neighbour chunk does not exist the halo is some finite value
+    // and the coupling multiplying it is zero (the caller zeroes the
+    // out-of-range couplings of the first and last chunk).
     double send[4] = {x[0], x[1], x[M - 2], x[M - 1]}, hp[4], hn[4];
-    ex.template swap<4>(send, hp, hn, 1);
-    double dxm = 0.0;
+    ex.template swapc<4>(send, hp, hn, 1);
+    float dxm = 0.0f;
     A.refresh();
 #pragma unroll
     for (int i = 0; i < M; ++i) {
@@ -317,23 +329,23 @@ __device__ __forceinline__ int ir_solve(const Cf &A, const Bv &b, double (&x)[M]
       const double xp1 = (i + 1 < M) ? x[i + 1] : hn[0];
       const double xp2 = (i + 2 < M) ? x[i + 2] : hn[i + 2 - M];
       double acc = (double)A.d(i) * x[i];
-      acc += (first && i < 2) ? 0.0 : (double)A.s(i) * xm2;
-      acc += (first && i < 1) ? 0.0 : (double)A.l(i) * xm1;
-      acc += (last && i >= M - 1) ? 0.0 : (double)A.u(i) * xp1;
-      acc += (last && i >= M - 2) ? 0.0 : (double)A.w(i) * xp2;
+      acc += (double)A.s(i) * xm2;
+      acc += (double)A.l(i) * xm1;
+      acc += (double)A.u(i) * xp1;
+      acc += (double)A.w(i) * xp2;
       f[i] = (float)(b(i) - acc);
     }
     ir_apply<M, MAXR>(F, A, f, p, P, ex);
 #pragma unroll
     for (int i = 0; i < M; ++i) {
       x[i] += (double)f[i];
-      dxm = fmax(dxm, fabs((double)f[i]));
+      dxm = fmaxf(dxm, fabsf(f[i]));
     }
     ++k;
     // Stop once the estimated relative forward error of the new iterate,
     // |dx_k|^2 / (|dx_{k-1}| |x|) (contraction |dx_k|/|dx_{k-1}| times the
     // error |dx_k| it removed), is at most tol; norms are system maxima.
-    const double dxs = max_all(ex, dxm, P);
+    const double dxs = (double)ex.max_f32(dxm);
     const bool conv = (dxs == 0.0) || (dxs * dxs <= tol * xnorm * dprev);
     dprev = dxs;
     if (ex.all(conv)) break;

@@ -32,8 +32,10 @@ __device__ __forceinline__ void note_steps(int *steps_out, int k) {
 // The coefficients live in LDS ([5][M][64] per wavefront) for the whole
 // refinement; the RHS is re-read from memory for each residual (the RHS is
 // not written before the final store, in place or not).
+// Occupancy: 3 waves per SIMD (<= 168 VGPRs) where the state fits without
+// spilling (M <= 8), else 2.
 template <int M, int R, bool ROT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
     k_ir_x(const float *__restrict__ ds, const float *__restrict__ dl,
            const float *__restrict__ d, const float *__restrict__ du,
            const float *__restrict__ dw, const double *bin,
@@ -58,6 +60,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       RowsIO<float, M>::load(src[k] + offc, t);
+      ir_zero_out_of_range<M>(k, t, p == 0, p == L - 1);
 #pragma unroll
       for (int i = 0; i < M; ++i) alds[(k * M + i) * 64] = t[i];
     }
@@ -68,8 +71,8 @@ __global__ void __launch_bounds__(256)
   const MemRhs b{bin + off, 1};
   const int k = ir_solve<M, R>(A, b, xs, p, L, ex, max_steps, tol);
   if constexpr (ROT) {
-    stage_rotated_store<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, L,
-                                   len, out, n_sys, blk, map);
+    stage_rotated_store<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, R,
+                                   (int)len, out, n_sys, blk, map);
   } else {
     if (valid) RowsIO<double, M>::store(out + off, xs);
   }
@@ -103,6 +106,10 @@ __global__ void __launch_bounds__(MAXT)
     w[i] = dw[e];
     b[i] = x[e];
   }
+  ir_zero_out_of_range<M>(0, s, p == 0, p == P - 1);
+  ir_zero_out_of_range<M>(1, l, p == 0, p == P - 1);
+  ir_zero_out_of_range<M>(3, u, p == 0, p == P - 1);
+  ir_zero_out_of_range<M>(4, w, p == 0, p == P - 1);
   LdsEx ex{smem_raw, sl, p, S, P};
   const RegCoef<M> A{s, l, dd, u, w};
   const RegRhs<M> bv{b};
@@ -216,15 +223,25 @@ template <int M, int R>
 hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot,
                    int64_t n_sys, int64_t len, const XMap &map, int max_steps,
                    double tol, int *steps, hipStream_t st) {
-  const int block = 256;
+#ifndef PENTA_ROT_BLOCK
+#define PENTA_ROT_BLOCK 256
+#endif
+  const int block = rot ? PENTA_ROT_BLOCK : 256;
   const int64_t threads = n_sys << R;
   const int64_t grid = (threads + block - 1) / block;
   const size_t smem_a = (size_t)(block / 64) * 5 * M * 64 * sizeof(float);
   if (rot) {
     const size_t tile = rotated_tile_bytes<double>(block, M, len);
-    hipLaunchKernelGGL((k_ir_x<M, R, true>), dim3((unsigned)grid), dim3(block),
-                       smem_a > tile ? smem_a : tile, st, a.ds, a.dl, a.d, a.du, a.dw,
-                       b, out, n_sys, len, max_steps, tol, steps, map);
+    const size_t smem = smem_a > tile ? smem_a : tile;
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&k_ir_x<M, R, true>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_ir_x<M, R, true>), dim3((unsigned)grid), dim3(block), smem, st,
+                       a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol,
+                       steps, map);
   } else {
     hipLaunchKernelGGL((k_ir_x<M, R, false>), dim3((unsigned)grid), dim3(block), smem_a,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps,

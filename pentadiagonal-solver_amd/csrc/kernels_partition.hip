@@ -29,7 +29,7 @@ namespace pentad {
 //              Workgroups are dealt to XCDs so that workgroups with adjacent
 //              systems (whose runs share 128-byte lines) use one L2.
 template <typename T, int M, bool ROT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
     k_part_x(const T *__restrict__ ds, const T *__restrict__ dl,
              const T *__restrict__ d, const T *__restrict__ du,
              const T *__restrict__ dw, const T *b, T *out, int64_t n_sys,
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256)
   part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
   if constexpr (ROT) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    stage_rotated_store<T, M>(reinterpret_cast<T *>(smem_raw), f, p, L, len, out,
+    stage_rotated_store<T, M>(reinterpret_cast<T *>(smem_raw), f, p, lshift, (int)len, out,
                               n_sys, blk, map);
   } else {
     if (valid) RowsIO<T, M>::store(out + off, f);
@@ -113,14 +113,23 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                 int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
-  const int block = 256;
+#ifndef PENTA_ROT_BLOCK
+#define PENTA_ROT_BLOCK 256
+#endif
+  const int block = rot ? PENTA_ROT_BLOCK : 256;
   const int64_t threads = n_sys * lanes;
   const int64_t grid = (threads + block - 1) / block;
-  if (rot)
-    hipLaunchKernelGGL((k_part_x<T, M, true>), dim3((unsigned)grid), dim3(block),
-                       rotated_tile_bytes<T>(block, M, len), st, a.ds, a.dl, a.d,
-                       a.du, a.dw, b, out, n_sys, len, lshift, map);
-  else
+  if (rot) {
+    const size_t smem = rotated_tile_bytes<T>(block, M, len);
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&k_part_x<T, M, true>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_part_x<T, M, true>), dim3((unsigned)grid), dim3(block), smem,
+                       st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+  } else
     hipLaunchKernelGGL((k_part_x<T, M, false>), dim3((unsigned)grid), dim3(block), 0,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
   return hipGetLastError();
