@@ -223,10 +223,7 @@ template <int M, int R>
 hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot,
                    int64_t n_sys, int64_t len, const XMap &map, int max_steps,
                    double tol, int *steps, hipStream_t st) {
-#ifndef PENTA_ROT_BLOCK
-#define PENTA_ROT_BLOCK 256
-#endif
-  const int block = rot ? PENTA_ROT_BLOCK : 256;
+  const int block = 256; // 512 measured slower (fewer resident workgroups)
   const int64_t threads = n_sys << R;
   const int64_t grid = (threads + block - 1) / block;
   const size_t smem_a = (size_t)(block / 64) * 5 * M * 64 * sizeof(float);

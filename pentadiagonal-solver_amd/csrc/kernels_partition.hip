@@ -28,6 +28,8 @@ namespace pentad {
 //              through LDS so that each store instruction writes runs.
 //              Workgroups are dealt to XCDs so that workgroups with adjacent
 //              systems (whose runs share 128-byte lines) use one L2.
+// (No occupancy hint: forcing FP64 rows=8 down to 128 VGPRs for 4 waves per
+// SIMD measured 8% slower than the compiler's 138 VGPRs at 3 waves.)
 template <typename T, int M, bool ROT>
 __global__ void __launch_bounds__(512)
     k_part_x(const T *__restrict__ ds, const T *__restrict__ dl,
@@ -113,10 +115,10 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                 int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
-#ifndef PENTA_ROT_BLOCK
-#define PENTA_ROT_BLOCK 256
-#endif
-  const int block = rot ? PENTA_ROT_BLOCK : 256;
+  // FP32 rotated stores: 512-thread workgroups own twice the systems, so each
+  // row of the tile is a 64-byte run (measured ~12% faster than 256 at 512^3;
+  // FP64 is faster at 256, where the state allows 3 waves per SIMD)
+  const int block = (rot && sizeof(T) == 4) ? 512 : 256;
   const int64_t threads = n_sys * lanes;
   const int64_t grid = (threads + block - 1) / block;
   if (rot) {

@@ -26,9 +26,14 @@
 // read into the arithmetic, matching the reference's don't-care convention.
 //
 // The exchange between the lanes of one system is abstracted by Ex:
-//   Ex::template swap<K>(const T *send, T *from_prev, T *from_next, int h)
-// delivers the K values of lane p-h and of lane p+h (zeros when out of
-// range).  Wavefront shuffles implement it for x, LDS for y/z.
+//   Ex::template swapc<K>(const T *send, T *from_prev, T *from_next, int h)
+// delivers the K values of lane p-h and of lane p+h; where that chunk does
+// not exist the value is a finite stand-in, and every use multiplies it by
+// an exact zero: the first chunk's block A and spikes vanish (it has no
+// previous separator), the last chunk's block C and separator couplings
+// past the end vanish, and block PCR keeps A = 0 for p < h and C = 0 for
+// p + h >= P in every round.  Wavefront DPP/shuffles implement it for x,
+// LDS for y/z.
 #pragma once
 
 #include "pentad_device.hpp"
@@ -148,7 +153,7 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
       send[c] = XF0[c];
       send[5 + c] = XF1[c];
     }
-    ex.template swap<10>(send, rp, rn, 1);
+    ex.template swapc<10>(send, rp, rn, 1);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       Y0[c] = rn[c];
@@ -194,12 +199,13 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
   }
 
   // ---- block parallel cyclic reduction -----------------------------------
-  for (int h = 1; h < P; h <<= 1) {
+  // (round h = 1 peeled so that its exchange is a compile-time lane shift)
+  auto round = [&](int h) {
     T send[10], rp[10], rn[10];
     send[0] = A.a00; send[1] = A.a01; send[2] = A.a10; send[3] = A.a11;
     send[4] = C.a00; send[5] = C.a01; send[6] = C.a10; send[7] = C.a11;
     send[8] = r0;    send[9] = r1;
-    ex.template swap<10>(send, rp, rn, h);
+    ex.template swapc<10>(send, rp, rn, h);
     const M2<T> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
     const M2<T> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
     // B' = I - A Cp - C An ; A' = -A Ap ; C' = -C Cn ; r' = r - A rp - C rn
@@ -218,14 +224,16 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
     C = mul(Bi, nC);
     r0 = Bi.a00 * q0 + Bi.a01 * q1;
     r1 = Bi.a10 * q0 + Bi.a11 * q1;
-  }
+  };
+  if (P > 1) round(1);
+  for (int h = 2; h < P; h <<= 1) round(h);
   const T t0 = r0, t1 = r1; // S_p
 
   // ---- 4. previous separator, final backward sweep -----------------------
   T s0, s1;
   {
     T send[2] = {t0, t1}, rp[2], rn[2];
-    ex.template swap<2>(send, rp, rn, 1);
+    ex.template swapc<2>(send, rp, rn, 1);
     s0 = rp[0];
     s1 = rp[1];
   }

@@ -1,11 +1,11 @@
 // partition_exchange.hpp -- how the lanes that share one system talk.
 //
-// swap<K>(send, from_prev, from_next, h) gives every lane the K values of
-// the lane h chunks before it and h chunks after it in the same system
-// (zeros when that chunk does not exist).  swapc<K> is the same exchange
-// without the zeroing: where the chunk does not exist the lane receives a
-// finite value of its own wavefront (its own, or a neighbouring system's
-// chunk), for callers whose multipliers of those values are exactly zero.
+// swapc<K>(send, from_prev, from_next, h) gives every lane the K values of
+// the lane h chunks before it and h chunks after it in the same system.
+// Where that chunk does not exist the lane receives its own value (or 0 at
+// the edge of the wavefront) -- no zeroing selects: the callers multiply
+// those values by exact zeros.  Values never cross from one
+// system to another, so a system with non-finite data cannot spoil another.
 // all(pred) is true when pred holds on every lane that takes part in the
 // exchange (wavefront or workgroup), so loops around swap stay uniform.
 // max_f32(v) is the maximum of v over the chunks of the system.
@@ -21,20 +21,6 @@ struct WaveEx {
   int p;    // chunk index = lane position inside the system's lane group
   int P;    // lanes per system
   template <int K, typename T>
-  __device__ __forceinline__ void swap(const T *send, T *from_prev,
-                                       T *from_next, int h) const {
-    const bool hp = (p - h) >= 0, hn = (p + h) < P;
-    const int sp = hp ? lane - h : lane;
-    const int sn = hn ? lane + h : lane;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const T vp = __shfl(send[k], sp, 64);
-      const T vn = __shfl(send[k], sn, 64);
-      from_prev[k] = hp ? vp : T(0);
-      from_next[k] = hn ? vn : T(0);
-    }
-  }
-  template <int K, typename T>
   __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next,
                                         int h) const {
     if (h == 1) { // DPP wave shifts: no LDS round trip, edges of the wave get 0
@@ -42,6 +28,10 @@ struct WaveEx {
       for (int k = 0; k < K; ++k) {
         from_prev[k] = dpp_shift<T, 0x138>(send[k]); // wave_shr:1
         from_next[k] = dpp_shift<T, 0x130>(send[k]); // wave_shl:1
+        if (P < 64) { // several systems per wavefront: never read a neighbour system
+          from_prev[k] = (p == 0) ? send[k] : from_prev[k];
+          from_next[k] = (p == P - 1) ? send[k] : from_next[k];
+        }
       }
     } else {
       const int sp = (p - h) >= 0 ? lane - h : lane;
@@ -100,26 +90,6 @@ struct LdsEx {
   int s, p;           // system, chunk of this thread
   int S, P;
   template <int K, typename T>
-  __device__ __forceinline__ void swap(const T *send, T *from_prev,
-                                       T *from_next, int h) const {
-    T *lds = reinterpret_cast<T *>(raw);
-    const bool hp = (p - h) >= 0, hn = (p + h) < P;
-    const int PS = P * S;
-    __syncthreads(); // earlier readers are done with the buffer
-#pragma unroll
-    for (int k = 0; k < K; ++k) lds[k * PS + p * S + s] = send[k];
-    __syncthreads();
-    const int op = (hp ? p - h : p) * S + s;
-    const int on = (hn ? p + h : p) * S + s;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const T vp = lds[k * PS + op];
-      const T vn = lds[k * PS + on];
-      from_prev[k] = hp ? vp : T(0);
-      from_next[k] = hn ? vn : T(0);
-    }
-  }
-  template <int K, typename T>
   __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next,
                                         int h) const {
     T *lds = reinterpret_cast<T *>(raw);
@@ -148,16 +118,6 @@ struct LdsEx {
     return v;
   }
 };
-
-template <typename Ex>
-__device__ __forceinline__ double max_all(Ex &ex, double v, int P) {
-  for (int h = 1; h < P; h <<= 1) {
-    double a[1] = {v}, bp[1], bn[1];
-    ex.template swap<1>(a, bp, bn, h);
-    v = fmax(v, fmax(bp[0], bn[0]));
-  }
-  return v;
-}
 
 // Block index remap so that consecutive tiles land on the same XCD: the
 // hardware deals workgroups to the 8 XCDs round-robin (block b -> XCD b % 8),

@@ -48,10 +48,7 @@ __device__ __forceinline__ void stage_rotated_store(T *tile, const T (&v)[M], in
   __syncthreads();
   const int64_t sys0 = blk * S;
   constexpr int V = 16 / (int)sizeof(T);
-#ifndef PENTA_ROT_VEC
-#define PENTA_ROT_VEC 1
-#endif
-  const bool vec = PENTA_ROT_VEC && S >= V && map.out_lo == 1 && map.S1 % S == 0 &&
+  const bool vec = S >= V && map.out_lo == 1 && map.S1 % S == 0 &&
                    map.out_row % V == 0 && map.out_hi % V == 0 &&
                    ((uintptr_t)out & 15) == 0 && sys0 + S <= n_sys;
   if (vec) {
