@@ -141,6 +141,38 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
     return pl;
   }
 
+  if (want == "shared-fact") {
+    // opt-in only (every system is solved with system 0's coefficients, as
+    // in src/cuda/singlenode/pentad_cuda.cu:962-963); a length the kernels
+    // do not cover falls back to the automatic choice below.
+    static const int sf64[] = {8, 16, 6, 4};
+    static const int sf32[] = {16, 8, 6, 4};
+    const int *pref = is64 ? sf64 : sf32;
+    if (ln.dir == 0) {
+      bool al = true;
+      for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
+      for (int k = 0; al && k < 4; ++k) {
+        if (sf_x_supported(elem_bytes, pref[k], ln.len)) {
+          pl.kind = Plan::SfX;
+          pl.rows = pref[k];
+          pl.lanes = (int)(ln.len / pref[k]);
+          return pl;
+        }
+      }
+    } else {
+      for (int k = 0; k < 4; ++k) {
+        for (int S = 64; S >= 1; S >>= 1) {
+          if (sf_yz_supported(elem_bytes, pref[k], S, ln.len)) {
+            pl.kind = Plan::SfYZ;
+            pl.rows = pref[k];
+            pl.sys_per_wg = S;
+            return pl;
+          }
+        }
+      }
+    }
+  }
+
   if (ln.dir == 0) {
     bool al = true;
     for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
@@ -268,6 +300,8 @@ size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes) {
   if (pl.kind == Plan::IrLane) return ir_lane_scratch_bytes(ln);
   if (pl.kind == Plan::Naive)
     return (size_t)3 * (size_t)ln.n_sys * (size_t)ln.len * (size_t)elem_bytes;
+  if (pl.kind == Plan::SfX || pl.kind == Plan::SfYZ)
+    return sf_table_bytes(elem_bytes, pl.rows, ln.len);
   return 0;
 }
 
@@ -292,6 +326,13 @@ std::string plan_config(const Plan &pl) {
     return buf;
   case Plan::IrLane:
     return "ir naive";
+  case Plan::SfX:
+    std::snprintf(buf, sizeof buf, "shared-fact rows=%d lanes=%d", pl.rows, pl.lanes);
+    return buf;
+  case Plan::SfYZ:
+    std::snprintf(buf, sizeof buf, "shared-fact rows=%d sys_per_wg=%d", pl.rows,
+                  pl.sys_per_wg);
+    return buf;
   default:
     return "naive";
   }
@@ -310,6 +351,21 @@ hipError_t launch_plan(const Plan &pl, const Diags<T> &a, T *x, const Lines &ln,
   case Plan::Naive:
     if (buffer == nullptr) return hipErrorInvalidValue;
     return launch_thomas_lane<T>(a, x, ln, static_cast<T *>(buffer), st);
+  case Plan::SfX:
+  case Plan::SfYZ: {
+    if (buffer == nullptr) return hipErrorInvalidValue;
+    T *tab = static_cast<T *>(buffer);
+    const bool ok = pl.kind == Plan::SfX
+                        ? sf_x_supported((int)sizeof(T), pl.rows, ln.len)
+                        : sf_yz_supported((int)sizeof(T), pl.rows, pl.sys_per_wg, ln.len);
+    if (!ok) return hipErrorInvalidValue; // decline before anything is launched
+    hipError_t e = launch_sf_factor<T>(a, ln, pl.rows, tab, st);
+    if (e != hipSuccess) return e;
+    if (pl.kind == Plan::SfX)
+      return launch_sf_x_map<T>(tab, x, x, false, ln.n_sys, ln.len, xmap_inplace(ln),
+                                pl.rows, st);
+    return launch_sf_yz<T>(tab, x, ln, pl.rows, pl.sys_per_wg, st);
+  }
   default: return hipErrorInvalidValue;
   }
 }
@@ -358,8 +414,18 @@ pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
   hipStream_t st = c->stream;
   hipError_t err = hipSuccess;
 
-  if (pl.kind != Plan::Naive) err = launch_plan<T>(pl, a, x, ln, nullptr, st);
-  if (pl.kind != Plan::Naive && err != hipSuccess) {
+  const bool sf = pl.kind == Plan::SfX || pl.kind == Plan::SfYZ;
+  if (sf && buffer == nullptr) {
+    std::fprintf(stderr,
+                 "[penta] shared-fact along solvedim %d needs %zu bytes of scratch for its "
+                 "factor table (pentadsolver_*_gpsv_batch_buffer_size_ext); got NULL\n",
+                 solvedim, plan_scratch_bytes(pl, ln, (int)sizeof(T)));
+    return PENTAD_STATUS_INVALID_VALUE;
+  }
+  if (pl.kind != Plan::Naive) err = launch_plan<T>(pl, a, x, ln, buffer, st);
+  // (a declined shared-fact launch does not fall back: the caller's buffer
+  // is sized for the factor table, not for the one-lane kernel's scratch)
+  if (pl.kind != Plan::Naive && !sf && err != hipSuccess) {
     // an opt-in kernel that declines falls back, as the reference does
     // (src/cuda/singlenode/pentad_cuda.cu:17-37)
     if (debug_launch())
@@ -382,7 +448,8 @@ pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
                  solvedim, hipGetErrorString(err));
     return PENTAD_STATUS_EXECUTION_FAILED;
   }
-  note_kernel(c, ln.dir, pl.kind == Plan::Naive ? "naive" : "thomas-pcr",
+  note_kernel(c, ln.dir,
+              pl.kind == Plan::Naive ? "naive" : (sf ? "shared-fact" : "thomas-pcr"),
               plan_config(pl));
   return PENTAD_STATUS_SUCCESS;
 }

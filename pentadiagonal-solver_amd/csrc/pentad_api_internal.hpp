@@ -13,7 +13,9 @@ namespace pentad {
 
 // Which kernel a solve launches and with what shape parameters.
 struct Plan {
-  enum Kind { Naive = 0, PartX = 1, PartYZ = 2, IrLane = 3, IrX = 4, IrYZ = 5 } kind = Naive;
+  enum Kind {
+    Naive = 0, PartX = 1, PartYZ = 2, IrLane = 3, IrX = 4, IrYZ = 5, SfX = 6, SfYZ = 7
+  } kind = Naive;
   int rows = 0;       // rows per lane (partitioned kernels)
   int lanes = 0;      // lanes per system (x)
   int sys_per_wg = 0; // systems per workgroup (y/z)

@@ -101,6 +101,23 @@ hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
                           int sys_per_wg, hipStream_t st);
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
 
+// Shared factorisation (every system of the direction has system 0's
+// coefficients): k_sf_factor writes the factor table of system 0 into `tab`
+// (sf_table_bytes), then the solve kernels stream only right-hand sides.
+size_t sf_table_bytes(int elem_bytes, int rows, int64_t len);
+bool sf_x_supported(int elem_bytes, int rows, int64_t len);
+bool sf_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
+template <typename T>
+hipError_t launch_sf_factor(const Diags<T> &a, const Lines &ln, int rows, T *tab,
+                            hipStream_t st);
+// x-like lines through XMap; rot as in launch_part_x_map
+template <typename T>
+hipError_t launch_sf_x_map(const T *tab, const T *b, T *out, bool rot, int64_t n_sys,
+                           int64_t len, const XMap &map, int rows, hipStream_t st);
+template <typename T>
+hipError_t launch_sf_yz(const T *tab, T *x, const Lines &ln, int rows, int sys_per_wg,
+                        hipStream_t st);
+
 // FP32 coefficients + FP64 RHS/solution with on-chip iterative refinement.
 // steps (device int, may be null) receives atomicMax of the refinement
 // steps taken.
