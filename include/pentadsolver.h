@@ -213,7 +213,12 @@ pentadStatus_t pentadsolver_probe_stream(pentadsolver_handle_t handle,
  * is back in the caller's layout.  x must be 16-byte aligned.
  *   mode 0: FP64 coefficients and field; mode 1: FP32 both;
  *   mode 2: FP32 coefficients, FP64 field, FP32 solve + FP64 refinement
- *           (max_refine, tol as in pentadsolver_DS_gpsv_batch_ir).
+ *           (max_refine, tol as in pentadsolver_DS_gpsv_batch_ir);
+ *   mode 3 / 4: shared factorisation in FP64 / FP32 (the reference's
+ *           "shared-fact", src/cuda/singlenode/pentad_cuda.cu:908-1195):
+ *           each direction's system 0 is factored once at create and every
+ *           step streams only the field; coeff_elem_bytes must be 8 / 4,
+ *           and every length must be one the shared-fact kernels cover.
  *   coeffs: coeff_sets * 5 device pointers (ds, dl, d, du, dw per set) in
  *   the standard layout with elements of coeff_elem_bytes (4 or 8); one set
  *   serves every direction (as the reference's apps do), or ndim sets give

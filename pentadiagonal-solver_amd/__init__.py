@@ -296,7 +296,7 @@ def probe_stream(ins, out, fmas=0, handle=None):
         _hval(handle), arr, _ptr(out), out.numel(), out.element_size(), int(fmas)))
 
 
-ADI_FP64, ADI_FP32, ADI_IR = 0, 1, 2
+ADI_FP64, ADI_FP32, ADI_IR, ADI_SHARED_FP64, ADI_SHARED_FP32 = 0, 1, 2, 3, 4
 
 
 class Adi:
@@ -304,9 +304,11 @@ class Adi:
 
     mode: ADI_FP64 (float64 field and coefficients), ADI_FP32 (float32),
     ADI_IR (float32 coefficients solved in FP32 with FP64 refinement on a
-    float64 field).  coeff_sets: one (ds, dl, d, du, dw) tuple of device
-    tensors for every direction, or one tuple per direction; float32 or
-    float64 (converted as the mode needs).
+    float64 field), ADI_SHARED_FP64 / ADI_SHARED_FP32 (shared factorisation:
+    each direction's system 0 is factored once and applied to every line;
+    coefficient dtype must match).  coeff_sets: one (ds, dl, d, du, dw)
+    tuple of device tensors for every direction, or one tuple per direction;
+    float32 or float64 (converted as the mode needs, except shared modes).
     """
 
     def __init__(self, mode, dims, coeff_sets, handle=None, max_refine=5, tol=1e-13):

@@ -21,7 +21,9 @@
 // cover fall back to the in-place solves of the drop-in API.
 //
 // Modes: 0 = FP64, 1 = FP32, 2 = FP32 coefficients with an FP64 field and
-// on-chip iterative refinement (pentadsolver_DS_gpsv_batch_ir semantics).
+// on-chip iterative refinement (pentadsolver_DS_gpsv_batch_ir semantics),
+// 3 / 4 = shared factorisation in FP64 / FP32: each direction's system 0 is
+// factored once into a table and the steps stream only the field.
 #include "pentad_api_internal.hpp"
 
 #include <algorithm>
@@ -74,6 +76,7 @@ struct Adi {
   bool rotating = true;
   void *coef[3][5] = {};
   void *coef_mem = nullptr;
+  void *tab[3] = {}; // shared-fact factor tables (modes 3, 4), in coef_mem
   Order corder[3]; // coefficient layout per direction
   void *field[2] = {nullptr, nullptr};
   void *field_mem = nullptr;
@@ -154,6 +157,12 @@ hipError_t solve_dir(Adi *A, int d, const void *in, void *out, bool first_step,
   const Plan &pl = A->plan[d];
   const int64_t len = A->dims[d], nsys = A->n / len;
   const XMap map = make_map(A, d, first_step, last_step);
+  if (A->mode == 3)
+    return launch_sf_x_map<double>((const double *)A->tab[d], (const double *)in,
+                                   (double *)out, true, nsys, len, map, pl.rows, st);
+  if (A->mode == 4)
+    return launch_sf_x_map<float>((const float *)A->tab[d], (const float *)in, (float *)out,
+                                  true, nsys, len, map, pl.rows, st);
   if (A->mode == 2) {
     const Diags<float> a{(const float *)A->coef[d][0], (const float *)A->coef[d][1],
                          (const float *)A->coef[d][2], (const float *)A->coef[d][3],
@@ -258,7 +267,9 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
                                        const void *const *coeffs,
                                        int coeff_sets, int coeff_elem_bytes,
                                        int max_refine, double tol) {
-  if (!out || !dims || !coeffs || mode < 0 || mode > 2 || ndim < 2 ||
+  const bool shared = (mode == 3 || mode == 4);
+  if (shared && coeff_elem_bytes != (mode == 3 ? 8 : 4)) return PENTAD_STATUS_INVALID_VALUE;
+  if (!out || !dims || !coeffs || mode < 0 || mode > 4 || ndim < 2 ||
       ndim > 3 || (coeff_sets != 1 && coeff_sets != ndim) ||
       (coeff_elem_bytes != 4 && coeff_elem_bytes != 8) || max_refine < 0)
     return PENTAD_STATUS_INVALID_VALUE;
@@ -276,8 +287,8 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
     A->dims[i] = dims[i];
     A->n *= dims[i];
   }
-  A->cbytes = (mode == 0) ? 8 : 4;
-  A->fbytes = (mode == 1) ? 4 : 8;
+  A->cbytes = (mode == 0 || mode == 3) ? 8 : 4;
+  A->fbytes = (mode == 1 || mode == 4) ? 4 : 8;
   hipStream_t st = c->stream;
   auto fail = [&](pentadStatus_t s) {
     free_adi(A);
@@ -294,6 +305,23 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
     ln.len = A->dims[d];
     ln.n_sys = A->n / ln.len;
     const void *aligned[6] = {};
+    if (shared) {
+      static const int sf64[] = {8, 16, 6, 4}, sf32[] = {16, 8, 6, 4};
+      for (int k = 0; k < 4 && A->plan[d].kind != Plan::SfX; ++k) {
+        const int r = (mode == 3 ? sf64 : sf32)[k];
+        if (sf_x_supported(A->cbytes, r, ln.len)) {
+          A->plan[d].kind = Plan::SfX;
+          A->plan[d].rows = r;
+          A->plan[d].lanes = (int)(ln.len / r);
+        }
+      }
+      if (A->plan[d].kind != Plan::SfX) {
+        std::fprintf(stderr, "[penta] adi: no shared-fact kernel for length %lld\n",
+                     (long long)ln.len);
+        return fail(PENTAD_STATUS_INVALID_VALUE);
+      }
+      continue;
+    }
     A->plan[d] = (mode == 2) ? plan_ir(c, ln, aligned) : plan_solve(c, ln, A->cbytes, aligned);
     if (A->plan[d].kind != Plan::PartX && A->plan[d].kind != Plan::IrX) A->rotating = false;
   }
@@ -302,6 +330,49 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
   for (int d = 0; d < ndim; ++d) A->corder[d] = A->rotating ? in_order(A, d, false) : Order();
   const size_t csz = (size_t)A->n * A->cbytes;
   const size_t fsz = (size_t)A->n * A->fbytes;
+  if (shared) {
+    size_t off[3] = {0, 0, 0}, total = 0;
+    for (int d = 0; d < ndim; ++d) {
+      off[d] = total;
+      total += (sf_table_bytes(A->cbytes, A->plan[d].rows, A->dims[d]) + 255) & ~(size_t)255;
+    }
+    if (hipMalloc(&A->coef_mem, total) != hipSuccess ||
+        hipMalloc(&A->field_mem, fsz * 2) != hipSuccess ||
+        hipMalloc((void **)&A->d_steps, sizeof(int)) != hipSuccess)
+      return fail(PENTAD_STATUS_ALLOC_FAILED);
+    A->field[0] = A->field_mem;
+    A->field[1] = (char *)A->field_mem + fsz;
+    if (launch_fill_int(A->d_steps, 0, st) != hipSuccess)
+      return fail(PENTAD_STATUS_EXECUTION_FAILED);
+    int64_t stride = 1;
+    for (int d = 0; d < ndim; ++d) {
+      A->tab[d] = (char *)A->coef_mem + off[d];
+      const void *const *cs = coeffs + (coeff_sets == 1 ? 0 : d) * 5;
+      for (int k = 0; k < 5; ++k)
+        if (!cs[k]) return fail(PENTAD_STATUS_INVALID_VALUE);
+      Lines ln; // system 0 along axis d of the caller's layout
+      ln.dir = 0;
+      ln.len = A->dims[d];
+      ln.n_sys = A->n / ln.len;
+      ln.stride = stride;
+      stride *= A->dims[d];
+      const hipError_t e =
+          mode == 3
+              ? launch_sf_factor<double>(Diags<double>{(const double *)cs[0], (const double *)cs[1],
+                                                       (const double *)cs[2], (const double *)cs[3],
+                                                       (const double *)cs[4]},
+                                         ln, A->plan[d].rows, (double *)A->tab[d], st)
+              : launch_sf_factor<float>(Diags<float>{(const float *)cs[0], (const float *)cs[1],
+                                                     (const float *)cs[2], (const float *)cs[3],
+                                                     (const float *)cs[4]},
+                                        ln, A->plan[d].rows, (float *)A->tab[d], st);
+      if (e != hipSuccess) return fail(PENTAD_STATUS_EXECUTION_FAILED);
+      A->cfg[d] = plan_config(A->plan[d]);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(PENTAD_STATUS_EXECUTION_FAILED);
+    *out = A;
+    return PENTAD_STATUS_SUCCESS;
+  }
   if (hipMalloc(&A->coef_mem, csz * 5 * ndim) != hipSuccess ||
       hipMalloc((void **)&A->d_steps, sizeof(int)) != hipSuccess)
     return fail(PENTAD_STATUS_ALLOC_FAILED);
@@ -358,7 +429,9 @@ pentadStatus_t pentadsolver_adi_step(pentadsolver_adi_t adi, void *x, int nsteps
   }
   if (A->rotating)
     for (int d = 0; d < A->ndim; ++d)
-      note_kernel(A->c, d, A->mode == 2 ? "ir" : "thomas-pcr", A->cfg[d]);
+      note_kernel(A->c, d,
+                  A->mode == 2 ? "ir" : (A->mode >= 3 ? "shared-fact" : "thomas-pcr"),
+                  A->cfg[d]);
   return PENTAD_STATUS_SUCCESS;
 }
 
