@@ -259,6 +259,36 @@ void pentadsolver_adi_destroy(pentadsolver_adi_t *adi);
  * along dir launched, e.g. "thomas-pcr rows=8 lanes=32". */
 const char *pentadsolver_kernel_config(pentadsolver_handle_t handle, int dir);
 
+/* Variable-length batch: n_sys independent systems packed back to back,
+ * system k owning elements offsets[k] .. offsets[k+1]-1 of every array
+ * (offsets: host array of n_sys + 1 entries; lengths 3 .. 8192).  create
+ * builds a device schedule once (systems sorted longest first and grouped
+ * into work items whose lanes per system follow the length); each solve is
+ * then one persistent launch on the handle's stream with no allocation and
+ * no host synchronisation (capturable in a hipGraph).  The plan keeps the
+ * handle, which must outlive it.  No reference counterpart (the reference
+ * batches systems of one length only). */
+typedef void *pentadsolver_varlen_t;
+pentadStatus_t pentadsolver_varlen_create(pentadsolver_handle_t handle,
+                                          pentadsolver_varlen_t *plan,
+                                          const int64_t *offsets, int64_t n_sys);
+void pentadsolver_varlen_destroy(pentadsolver_varlen_t *plan);
+/* number of work items of the schedule */
+int pentadsolver_varlen_items(pentadsolver_varlen_t plan);
+pentadStatus_t pentadsolver_D_gpsv_varlen_batch(pentadsolver_varlen_t plan,
+                                                const double *ds, const double *dl,
+                                                const double *d, const double *du,
+                                                const double *dw, double *x);
+pentadStatus_t pentadsolver_S_gpsv_varlen_batch(pentadsolver_varlen_t plan,
+                                                const float *ds, const float *dl,
+                                                const float *d, const float *du,
+                                                const float *dw, float *x);
+/* FP32 coefficients, FP64 x; refinement as in pentadsolver_DS_gpsv_batch_ir */
+pentadStatus_t pentadsolver_DS_gpsv_varlen_batch_ir(
+    pentadsolver_varlen_t plan, const float *ds, const float *dl, const float *d,
+    const float *du, const float *dw, double *x, int max_refine, double tol,
+    int *d_refine_steps);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -59,6 +59,12 @@ _SIGNATURES = {
     "pentadsolver_adi_direction_ms": (_I, [_P, _P, _I, ctypes.POINTER(ctypes.c_float)]),
     "pentadsolver_adi_config": (ctypes.c_char_p, [_P, _I]),
     "pentadsolver_adi_destroy": (None, [ctypes.POINTER(_P)]),
+    "pentadsolver_varlen_create": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]),
+    "pentadsolver_varlen_destroy": (None, [ctypes.POINTER(_P)]),
+    "pentadsolver_varlen_items": (_I, [_P]),
+    "pentadsolver_D_gpsv_varlen_batch": (_I, [_P] * 7),
+    "pentadsolver_S_gpsv_varlen_batch": (_I, [_P] * 7),
+    "pentadsolver_DS_gpsv_varlen_batch_ir": (_I, [_P] * 7 + [_I, ctypes.c_double, _P]),
 }
 
 STATUS_NAMES = {
@@ -369,3 +375,59 @@ class Adi:
 def kernel_that_ran(direction: int) -> str:
     """Reference pentadsolver_kernel_that_ran (pentad_cuda.cu:1159-1161)."""
     return load_library().pentadsolver_kernel_that_ran(direction).decode()
+
+
+class Varlen:
+    """Variable-length batch (pentadsolver_varlen_*): systems packed back to
+    back, system k owning elements offsets[k]..offsets[k+1]-1.
+
+    offsets: host sequence of n_sys + 1 int64 values.  solve(ds, dl, d, du,
+    dw, x) dispatches on the dtypes: float64 -> FP64, float32 -> FP32,
+    float32 coefficients with a float64 x -> FP32 solve + FP64 refinement
+    (max_refine, tol, steps as in gpsv_batch_ir).
+    """
+
+    def __init__(self, offsets, handle=None):
+        import numpy as _np
+        lib = load_library()
+        off = _np.ascontiguousarray(offsets, dtype=_np.int64)
+        self._plan = _P()
+        self._handle = handle  # the plan uses the handle: keep it alive
+        _check(lib.pentadsolver_varlen_create(
+            _hval(handle), ctypes.byref(self._plan),
+            off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(off) - 1))
+
+    def items(self) -> int:
+        return int(load_library().pentadsolver_varlen_items(self._plan))
+
+    def solve(self, ds, dl, d, du, dw, x, max_refine=5, tol=1e-13, steps=None):
+        import torch
+        lib = load_library()
+        cdt = ds.dtype
+        args = [self._plan] + [_ptr(t) for t in (ds, dl, d, du, dw, x)]
+        if cdt == torch.float32 and x.dtype == torch.float64:
+            _check(lib.pentadsolver_DS_gpsv_varlen_batch_ir(*args, int(max_refine), float(tol),
+                                                            _ptr(steps)))
+        elif cdt == x.dtype == torch.float64:
+            _check(lib.pentadsolver_D_gpsv_varlen_batch(*args))
+        elif cdt == x.dtype == torch.float32:
+            _check(lib.pentadsolver_S_gpsv_varlen_batch(*args))
+        else:
+            raise TypeError("varlen: float64, float32, or float32 coefficients with float64 x")
+
+    def close(self):
+        if self._plan:
+            load_library().pentadsolver_varlen_destroy(ctypes.byref(self._plan))
+            self._plan = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

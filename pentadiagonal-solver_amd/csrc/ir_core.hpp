@@ -84,13 +84,18 @@ __device__ __forceinline__ void ir_zero_out_of_range(int k, float (&t)[M], bool 
 
 // Factorisation of one chunk: du2, 1/pivot, the previous-separator spikes
 // (za, zb) and the block PCR multipliers of each round (MAXR >= ceil(log2 P)).
+// MAXR = 0 stores only the normalised first block row (A0, C0) and redoes
+// the block PCR inside every apply: 12 registers per round are too many for
+// long systems (hundreds of chunks), and the rounds cost arithmetic only.
 // The separator rows' couplings are read from the coefficients when used
 // (the last chunk's out-of-range ones are zero there).
 template <int M, int MAXR> struct IrFactors {
   static constexpr int m = M - 2;
+  static constexpr int NR = MAXR > 0 ? MAXR : 1;
   float du2[m], iv[m], za[m], zb[m];
   M2<float> Bi0;
-  M2<float> Ah[MAXR], Ch[MAXR], Bi[MAXR];
+  M2<float> Ah[NR], Ch[NR], Bi[NR]; // MAXR > 0
+  M2<float> A0, C0;                 // MAXR == 0
 };
 
 // eliminated lower coupling of interior row i (ddl) and dw2 = w / pivot
@@ -200,6 +205,11 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
   F.Bi0 = inv(B);
   Am = mul(F.Bi0, Am);
   C = mul(F.Bi0, C);
+  if constexpr (MAXR == 0) {
+    F.A0 = Am;
+    F.C0 = C;
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < MAXR; ++j) {
     const int h = 1 << j;
@@ -256,6 +266,29 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
   const float r1 = f[m + 1] - A.s(m + 1) * xl1 - A.u(m + 1) * y0 - A.w(m + 1) * y1;
   float q0 = F.Bi0.a00 * r0 + F.Bi0.a01 * r1;
   float q1 = F.Bi0.a10 * r0 + F.Bi0.a11 * r1;
+  if constexpr (MAXR == 0) { // block PCR on (A, C, q) together, as in part_solve
+    M2<float> Am = F.A0, C = F.C0;
+    auto round = [&](int h) {
+      float send[10] = {Am.a00, Am.a01, Am.a10, Am.a11, C.a00, C.a01, C.a10, C.a11, q0, q1};
+      float rp[10], rn[10];
+      ex.template swapc<10>(send, rp, rn, h);
+      const M2<float> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
+      const M2<float> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
+      const M2<float> ACp = mul(Am, Cp), CAn = mul(C, An);
+      const M2<float> Bn{1.0f - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
+                         -ACp.a10 - CAn.a10, 1.0f - ACp.a11 - CAn.a11};
+      const M2<float> Bi = inv(Bn);
+      const float t0 = q0 - (Am.a00 * rp[8] + Am.a01 * rp[9]) - (C.a00 * rn[8] + C.a01 * rn[9]);
+      const float t1 = q1 - (Am.a10 * rp[8] + Am.a11 * rp[9]) - (C.a10 * rn[8] + C.a11 * rn[9]);
+      const M2<float> AA = mul(Am, Ap), CC = mul(C, Cn);
+      Am = mul(Bi, M2<float>{-AA.a00, -AA.a01, -AA.a10, -AA.a11});
+      C = mul(Bi, M2<float>{-CC.a00, -CC.a01, -CC.a10, -CC.a11});
+      q0 = Bi.a00 * t0 + Bi.a01 * t1;
+      q1 = Bi.a10 * t0 + Bi.a11 * t1;
+    };
+    if (P > 1) round(1);
+    for (int h = 2; h < P; h <<= 1) round(h);
+  }
 #pragma unroll
   for (int j = 0; j < MAXR; ++j) {
     const int h = 1 << j;

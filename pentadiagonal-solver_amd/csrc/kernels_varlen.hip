@@ -1,0 +1,227 @@
+// kernels_varlen.hip -- many systems of different lengths in one launch.
+//
+// Systems are packed back to back: system k owns rows offs[k]..offs[k+1]-1
+// of every array.  The host sorts the systems by length (longest first) and
+// cuts the sorted list into work items; an item is S systems that share one
+// chunk count P (a power of two, P * M >= the longest length of the item,
+// S * P = VL_BLOCK threads).  Thread t of the workgroup owns chunk
+// p = t % P of system t / P: the chunks of one system are consecutive
+// lanes, so its rows are read as one contiguous span.  Rows past a system's
+// length are virtual identity rows (d = 1, no couplings, zero right-hand
+// side), which leaves the real rows' solution unchanged; couplings of real
+// rows past the end are zeroed.  The workgroups are persistent: workgroup b
+// takes items b, b + G, b + 2G, ... of the longest-first list.
+//
+// Exchanges: P <= 64 keeps a system inside one wavefront (DPP / shuffles);
+// longer systems exchange through LDS across the workgroup.
+#include "ir_core.hpp"
+#include "partition_core.hpp"
+#include "partition_exchange.hpp"
+#include "pentad_internal.hpp"
+
+namespace pentad {
+
+// exchange through LDS between the chunks of one system that are the
+// consecutive threads [sys * P, sys * P + P) of the workgroup
+struct VarEx {
+  unsigned char *raw; // K * N elements
+  int t, p, P, N;
+  template <int K, typename T>
+  __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next,
+                                        int h) const {
+    T *lds = reinterpret_cast<T *>(raw);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[k * N + t] = send[k];
+    __syncthreads();
+    const int op = (p - h) >= 0 ? t - h : t;
+    const int on = (p + h) < P ? t + h : t;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      from_prev[k] = lds[k * N + op];
+      from_next[k] = lds[k * N + on];
+    }
+  }
+  __device__ __forceinline__ bool all(bool pred) const {
+    return __syncthreads_and(pred) != 0;
+  }
+  __device__ __forceinline__ float max_f32(float v) const {
+    for (int h = 1; h < P; h <<= 1) {
+      float a[1] = {v}, bp[1], bn[1];
+      swapc<1>(a, bp, bn, h);
+      v = fmaxf(v, fmaxf(bp[0], bn[0]));
+    }
+    return v;
+  }
+};
+
+// right-hand side rows of a chunk; rows past the end of the system are 0
+struct VarRhs {
+  const double *p;
+  int nv; // real rows in this chunk
+  __device__ __forceinline__ double operator()(int i) const { return i < nv ? p[i] : 0.0; }
+};
+
+namespace {
+
+struct Item {
+  int first, count, pshift;
+};
+__device__ __forceinline__ Item item_at(const int2 *items, int it) {
+  const int2 v = items[it];
+  return Item{v.x, v.y & 0xFFFF, v.y >> 16};
+}
+
+// rows [r0, r0 + M) of a system of length len starting at element off
+template <typename T, typename C, int M>
+__device__ __forceinline__ void load_rows(const C *__restrict__ ds, const C *__restrict__ dl,
+                                          const C *__restrict__ d, const C *__restrict__ du,
+                                          const C *__restrict__ dw, int64_t off, int64_t r0,
+                                          int64_t len, T (&s)[M], T (&l)[M], T (&dd)[M],
+                                          T (&u)[M], T (&w)[M]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int64_t r = r0 + i, e = off + r;
+    const bool in = r < len;
+    s[i] = (in && r >= 2) ? (T)ds[e] : T(0);
+    l[i] = (in && r >= 1) ? (T)dl[e] : T(0);
+    dd[i] = in ? (T)d[e] : T(1);
+    u[i] = (r + 1 < len) ? (T)du[e] : T(0);
+    w[i] = (r + 2 < len) ? (T)dw[e] : T(0);
+  }
+}
+
+} // namespace
+
+template <typename T, int M>
+__global__ void __launch_bounds__(VL_BLOCK)
+    k_varlen(const T *__restrict__ ds, const T *__restrict__ dl, const T *__restrict__ d,
+             const T *__restrict__ du, const T *__restrict__ dw, T *__restrict__ x,
+             const int64_t *__restrict__ offs, const int *__restrict__ perm,
+             const int2 *__restrict__ items, int n_items) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int t = threadIdx.x;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const Item I = item_at(items, it);
+    const int P = 1 << I.pshift;
+    const int sl = t >> I.pshift, p = t & (P - 1);
+    const bool active = sl < I.count;
+    int64_t off = 0, len = 0;
+    if (active) {
+      const int sys = perm[I.first + sl];
+      off = offs[sys];
+      len = offs[sys + 1] - off;
+    }
+    const int64_t r0 = (int64_t)p * M;
+    T s[M], l[M], dd[M], u[M], w[M], f[M];
+    load_rows<T, T, M>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
+#pragma unroll
+    for (int i = 0; i < M; ++i) f[i] = (r0 + i < len) ? x[off + r0 + i] : T(0);
+    if (P <= 64) {
+      WaveEx ex{t & 63, p, P};
+      part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
+    } else {
+      VarEx ex{smem_raw, t, p, P, VL_BLOCK};
+      part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+      if (r0 + i < len) x[off + r0 + i] = f[i];
+  }
+}
+
+// FP32 coefficients, FP64 right-hand side / solution, refinement on chip
+// (ir_core.hpp with the block PCR redone in every apply).
+template <int M>
+__global__ void __launch_bounds__(VL_BLOCK)
+    k_varlen_ir(const float *__restrict__ ds, const float *__restrict__ dl,
+                const float *__restrict__ d, const float *__restrict__ du,
+                const float *__restrict__ dw, double *x, const int64_t *__restrict__ offs,
+                const int *__restrict__ perm, const int2 *__restrict__ items, int n_items,
+                int max_steps, double tol, int *__restrict__ steps_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int t = threadIdx.x;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const Item I = item_at(items, it);
+    const int P = 1 << I.pshift;
+    const int sl = t >> I.pshift, p = t & (P - 1);
+    const bool active = sl < I.count;
+    int64_t off = 0, len = 0;
+    if (active) {
+      const int sys = perm[I.first + sl];
+      off = offs[sys];
+      len = offs[sys + 1] - off;
+    }
+    const int64_t r0 = (int64_t)p * M;
+    float s[M], l[M], dd[M], u[M], w[M];
+    load_rows<float, float, M>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
+    const int64_t nv = len - r0;
+    const VarRhs b{x + off + r0, nv <= 0 ? 0 : (nv >= M ? M : (int)nv)};
+    const RegCoef<M> A{s, l, dd, u, w};
+    double xs[M];
+    int k;
+    if (P <= 64) {
+      WaveEx ex{t & 63, p, P};
+      k = ir_solve<M, 0>(A, b, xs, p, P, ex, max_steps, tol);
+    } else {
+      VarEx ex{smem_raw, t, p, P, VL_BLOCK};
+      k = ir_solve<M, 0>(A, b, xs, p, P, ex, max_steps, tol);
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+      if (r0 + i < len) x[off + r0 + i] = xs[i];
+    if (active && p == 0 && steps_out != nullptr) {
+      const int cur = __hip_atomic_load(steps_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k > cur) atomicMax(steps_out, k);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+int64_t vl_grid(int n_items) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        v > 0)
+      cus = v;
+  }
+  const int64_t cap = (int64_t)cus * 2;
+  return n_items < cap ? n_items : cap;
+}
+
+} // namespace
+
+template <typename T>
+hipError_t launch_varlen(const Diags<T> &a, T *x, const int64_t *offs, const int *perm,
+                         const int2 *items, int n_items, hipStream_t st) {
+  if (n_items <= 0) return hipSuccess;
+  const size_t smem = (size_t)10 * VL_BLOCK * sizeof(T);
+  hipLaunchKernelGGL((k_varlen<T, VL_ROWS>), dim3((unsigned)vl_grid(n_items)),
+                     dim3(VL_BLOCK), smem, st, a.ds, a.dl, a.d, a.du, a.dw, x, offs, perm,
+                     items, n_items);
+  return hipGetLastError();
+}
+
+hipError_t launch_varlen_ir(const Diags<float> &a, double *x, const int64_t *offs,
+                            const int *perm, const int2 *items, int n_items, int max_steps,
+                            double tol, int *steps, hipStream_t st) {
+  if (n_items <= 0) return hipSuccess;
+  // largest exchange: 10 floats or 4 doubles per thread
+  const size_t smem = (size_t)10 * VL_BLOCK * sizeof(float);
+  hipLaunchKernelGGL((k_varlen_ir<VL_ROWS>), dim3((unsigned)vl_grid(n_items)), dim3(VL_BLOCK),
+                     smem, st, a.ds, a.dl, a.d, a.du, a.dw, x, offs, perm, items, n_items,
+                     max_steps, tol, steps);
+  return hipGetLastError();
+}
+
+template hipError_t launch_varlen<float>(const Diags<float> &, float *, const int64_t *,
+                                         const int *, const int2 *, int, hipStream_t);
+template hipError_t launch_varlen<double>(const Diags<double> &, double *, const int64_t *,
+                                          const int *, const int2 *, int, hipStream_t);
+
+} // namespace pentad

@@ -101,6 +101,18 @@ hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
                           int sys_per_wg, hipStream_t st);
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
 
+// Variable-length batch: systems packed back to back (offsets), solved in
+// one persistent launch over work items of S systems x P chunks of
+// VL_ROWS rows (S * P = VL_BLOCK); lengths 3 .. VL_BLOCK * VL_ROWS.
+constexpr int VL_BLOCK = 512;
+constexpr int VL_ROWS = 16;
+template <typename T>
+hipError_t launch_varlen(const Diags<T> &a, T *x, const int64_t *offs, const int *perm,
+                         const int2 *items, int n_items, hipStream_t st);
+hipError_t launch_varlen_ir(const Diags<float> &a, double *x, const int64_t *offs,
+                            const int *perm, const int2 *items, int n_items, int max_steps,
+                            double tol, int *steps, hipStream_t st);
+
 // Shared factorisation (every system of the direction has system 0's
 // coefficients): k_sf_factor writes the factor table of system 0 into `tab`
 // (sf_table_bytes), then the solve kernels stream only right-hand sides.
