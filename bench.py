@@ -21,6 +21,16 @@ one step is re-run from the initial field and compared on the full grid with
 the CPU oracle: FP64 against the FP64 oracle, FP32+refinement against the
 FP64 oracle of the same FP32-stored coefficients widened to FP64.
 
+Beside the headline line's own fields, "weak_set" repeats the refinement
+step on the weakly dominant set (c = 1.05 sum|off|), and "workloads" holds
+the other configurations, each timed and checked against the oracle:
+256^3 x 50 iterations (FP64, FP32, and the constant-coefficient shared-fact
+variant; FP64 also against the reference's own CPU solver), 8192 systems of
+log-uniform length in [32, 8192] in one launch vs one call per system, and
+the 384x256x640 hyperdiffusion run (100 steps as one hipGraph, FP32+FP64
+refinement vs FP64, the first --c5-cpu-steps steps also on the CPU).
+--no-extra skips them.
+
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--size 512]
 """
 from __future__ import annotations
@@ -141,7 +151,7 @@ def oracle_step(coeffs_np, b_np, dims):
     return x
 
 
-def cpu_baseline(coeffs_np, b_np, dims, iters=2):
+def cpu_baseline(coeffs_np, b_np, dims, iters=5):
     """The reference's own OpenMP+AVX2 CPU solver (oracle/_ref, built from its
     sources) timed on this host, FP64, method of apps/adi_cpu.cpp:49-103."""
     from oracle import oracle as O
@@ -173,6 +183,260 @@ def cpu_baseline(coeffs_np, b_np, dims, iters=2):
                       f"OMP_NUM_THREADS={threads}"}
 
 
+# ---------------------------------------------------------------------------
+# the other configurations: reported beside the headline, each with its check
+# ---------------------------------------------------------------------------
+def graph_ms_per_step(torch, adi, x, b, steps):
+    """Capture `steps` ADI steps on x as one hipGraph, replay once to warm,
+    then time one replay from the initial field (ms per step)."""
+    stream = torch.cuda.current_stream()
+    x.copy_(b)
+    adi.capture(x, steps)
+    adi.replay(1)
+    torch.cuda.synchronize()
+    x.copy_(b)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    adi.replay(1)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def max_rel(a, ref):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - ref)) / np.max(np.abs(ref)))
+
+
+def weak_set(P, torch, W, h, size, steps, max_refine, tol, check):
+    """Precision config, weakly dominant set: c = 1.05 * sum|off| (seed 4)."""
+    dims = [size] * 3
+    n = size ** 3
+    ds, dl, d, du, dw, b = W.dominant_torch(dims, 4, ratio=(1.05, 1.05), dtype=torch.float64)
+    c32 = tuple(a.float() for a in (ds, dl, d, du, dw))
+    del ds, dl, d, du, dw
+    x = b.clone()
+    with P.Adi(P.ADI_IR, dims, [c32], handle=h, max_refine=max_refine, tol=tol) as adi:
+        ms = graph_ms_per_step(torch, adi, x, b, steps)
+        adi.refine_steps()
+        x.copy_(b)
+        adi.step(x, 1)
+        steps_taken = adi.refine_steps()
+        cfg = [adi.config(k) for k in range(3)]
+    out = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg,
+           "refine_steps_one_step": steps_taken}
+    if check:
+        cw = [a.double().cpu().numpy() for a in c32]
+        ref = oracle_step(cw, b.cpu().numpy(), dims)
+        out["max_rel_err_vs_fp64_oracle"] = max_rel(x.cpu().numpy(), ref)
+    return out
+
+
+def workload_c2(P, torch, W, h, check, iters=50):
+    """256^3 drop-in config: 50 ADI iterations, FP64 and FP32, variable
+    coefficients (seed 2) and the constant-coefficient (shared-fact) variant;
+    FP64 compared with the reference's own CPU solver (oracle/_ref) and the
+    oracle after one step."""
+    from oracle import oracle as O
+    dims = [256] * 3
+    n = 256 ** 3
+    ds, dl, d, du, dw, b = W.dominant_torch(dims, 2, dtype=torch.float64)
+    c64 = (ds, dl, d, du, dw)
+    c32 = tuple(a.float() for a in c64)
+    res = {"grid": dims, "iterations": iters}
+    one = {}
+    for mode, cs, x0 in (("fp64", c64, b), ("fp32", c32, b.float())):
+        m = P.ADI_FP64 if mode == "fp64" else P.ADI_FP32
+        x = x0.clone()
+        with P.Adi(m, dims, [cs], handle=h) as adi:
+            ms = graph_ms_per_step(torch, adi, x, x0, iters)
+            x.copy_(x0)
+            adi.step(x, 1)
+            cfg = [adi.config(k) for k in range(3)]
+        one[mode] = x.double().cpu().numpy()
+        res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg}
+    # constant-coefficient variant: one profile per direction, replicated
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    prof_sets = []
+    for ax in range(3):
+        off = [torch.empty(256, dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=g)
+               for _ in range(4)]
+        r = torch.empty(256, dtype=torch.float64, device="cuda").uniform_(2, 3, generator=g)
+        dd = r * sum(o.abs() for o in off)
+        view = [1, 1, 1]
+        view[2 - ax] = 256
+        prof_sets.append([a.reshape(view).expand(256, 256, 256).contiguous().reshape(-1)
+                          for a in (off[0], off[1], dd, off[2], off[3])])
+    for mode, dt, m in (("shared_fp64", torch.float64, P.ADI_SHARED_FP64),
+                        ("shared_fp32", torch.float32, P.ADI_SHARED_FP32)):
+        sets = [[a.to(dt) for a in cs] for cs in prof_sets]
+        x0 = b.to(dt)
+        x = x0.clone()
+        with P.Adi(m, dims, sets, handle=h) as adi:
+            ms = graph_ms_per_step(torch, adi, x, x0, iters)
+            x.copy_(x0)
+            adi.step(x, 1)
+            cfg = [adi.config(k) for k in range(3)]
+        one[mode] = x.double().cpu().numpy()
+        res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg,
+                     "bytes_per_row": 16 if dt == torch.float64 else 8}
+    if check:
+        b_np = b.cpu().numpy()
+        c_np = [a.cpu().numpy() for a in c64]
+        ref = oracle_step(c_np, b_np, dims)
+        res["fp64"]["max_rel_err_vs_fp64_oracle"] = max_rel(one["fp64"], ref)
+        if O.ref_available("avx"):
+            xr = b_np.copy()
+            for sd in range(3):
+                O.ref_gpsv_batch(*c_np, xr, dims, sd)
+            res["fp64"]["max_rel_err_vs_reference_cpu"] = max_rel(one["fp64"], xr)
+        c32_np = [a.double().cpu().numpy() for a in c32]
+        ref32 = oracle_step(c32_np, b.float().double().cpu().numpy(), dims)
+        res["fp32"]["max_rel_err_vs_fp64_oracle"] = max_rel(one["fp32"], ref32)
+        for mode, dt in (("shared_fp64", torch.float64), ("shared_fp32", torch.float32)):
+            x = b.to(dt).double().cpu().numpy()
+            for sd in range(3):
+                cs = [a.to(dt).double().cpu().numpy() for a in prof_sets[sd]]
+                O.gpsv_batch(*cs, x, dims, sd)
+            res[mode]["max_rel_err_vs_fp64_oracle"] = max_rel(one[mode], x)
+    return res
+
+
+def workload_c4(P, torch, W, h, check, reps=10):
+    """Batching config: 8192 systems, lengths log-uniform in [32, 8192]
+    (seed 5), dominance ratio U(1.2, 3) (seed 6); one persistent launch per
+    solve vs one solve call per system (FP64)."""
+    from oracle import oracle as O
+    off = W.varlen_offsets(8192, 5)
+    n = int(off[-1])
+    rng = np.random.default_rng(6)
+    ds, dl, du, dw = (rng.uniform(-1, 1, n) for _ in range(4))
+    d = rng.uniform(1.2, 3.0, n) * (abs(ds) + abs(dl) + abs(du) + abs(dw))
+    b = rng.uniform(-1, 1, n)
+    c64 = [torch.from_numpy(a).cuda() for a in (ds, dl, d, du, dw)]
+    c32 = [t.float() for t in c64]
+    bt = torch.from_numpy(b).cuda()
+    stream = torch.cuda.current_stream()
+    res = {"systems": 8192, "rows": n}
+    with P.Varlen(off, handle=h) as vl:
+        res["work_items"] = vl.items()
+        for mode, cs, x0 in (("fp64", c64, bt), ("fp32", c32, bt.float()), ("ir", c32, bt)):
+            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode == "ir" else None
+            x = x0.clone()
+            vl.solve(*cs, x, steps=steps)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                vl.solve(*cs, x, steps=steps)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            x.copy_(x0)
+            vl.solve(*cs, x, steps=steps)
+            r = {"ms": ms, "grows": n / (ms * 1e-3) / 1e9}
+            if steps is not None:
+                steps.zero_()
+                x.copy_(x0)
+                vl.solve(*cs, x, steps=steps)
+                r["refine_steps_max"] = int(steps.item())
+            if check:
+                if mode == "fp64":
+                    ref = O.varlen(ds, dl, d, du, dw, b, off)
+                else:
+                    w = [a.cpu().numpy().astype(np.float64) for a in c32]
+                    rhs = b.astype(np.float32).astype(np.float64) if mode == "fp32" else b
+                    ref = O.varlen(*w, rhs, off)
+                r["max_rel_err_vs_fp64_oracle"] = max_rel(x.double().cpu().numpy(), ref)
+            res[mode] = r
+    # one solve call per system (FP64, in place, through the drop-in entry point)
+    x = bt.clone()
+    lens = np.diff(off)
+    scratch = torch.empty(3 * int(lens.max()) * 8, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(len(lens)):
+        lo, hi = int(off[k]), int(off[k + 1])
+        P.gpsv_batch(*[t[lo:hi] for t in c64], x[lo:hi], [hi - lo], 0, scratch, h, check=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    res["per_system_fp64"] = {"ms": ms, "grows": n / (ms * 1e-3) / 1e9,
+                              "note": "8192 gpsv_batch calls (1-D, one system each), wall clock"}
+    res["speedup_one_launch_vs_per_system_fp64"] = ms / res["fp64"]["ms"]
+    return res
+
+
+def workload_c5(P, torch, W, h, check, steps=100, cpu_steps=2, max_refine=5, tol=1e-13):
+    """Hyperdiffusion config: 384x256x640, I + nu dt/h^4 [1,-4,6,-4,1] per
+    direction with nu ~ U(0.5, 2) per grid line (seed 7), N(0,1) initial
+    field (seed 8), 100 steps as one hipGraph; FP32+FP64 refinement vs FP64."""
+    dims = [384, 256, 640]
+    n = int(np.prod(dims))
+    sets64 = W.hyperdiffusion_torch(dims, 7)
+    sets32 = [[a.float() for a in cs] for cs in sets64]
+    b = W.gaussian_field_torch(dims, 8)
+    res = {"grid": dims, "steps": steps, "dt_over_h4": W.HYPERDIFFUSION_DT_OVER_H4}
+    out = {}
+    for mode, sets, m in (("fp64", sets64, P.ADI_FP64), ("ir", sets32, P.ADI_IR)):
+        x = b.clone()
+        with P.Adi(m, dims, sets, handle=h, max_refine=max_refine, tol=tol) as adi:
+            ms = graph_ms_per_step(torch, adi, x, b, steps)
+            refine = adi.refine_steps() if mode == "ir" else None
+            x.copy_(b)
+            adi.capture(x, steps)
+            adi.replay(1)
+            torch.cuda.synchronize()
+            cfg = [adi.config(k) for k in range(3)]
+        out[mode] = x.clone()
+        res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg}
+        if refine is not None:
+            res[mode]["refine_steps_max"] = refine
+    # the refinement run solves the FP32-stored operator: its solver error after
+    # all steps is measured against an FP64 run of that same (widened) operator
+    xw = b.clone()
+    with P.Adi(P.ADI_FP64, dims, [[a.double() for a in cs] for cs in sets32], handle=h) as adi:
+        adi.capture(xw, steps)
+        adi.replay(1)
+        torch.cuda.synchronize()
+    res["ir_max_rel_err_vs_fp64_same_operator_after_steps"] = float(
+        ((out["ir"] - xw).abs().max() / xw.abs().max()).item())
+    # and the FP32 storage of the operator itself moves the answer by:
+    res["fp32_operator_vs_fp64_operator_after_steps"] = float(
+        ((xw - out["fp64"]).abs().max() / out["fp64"].abs().max()).item())
+    del xw
+    res["energy_ratio_fp64"] = float((out["fp64"].norm() / b.norm()).item())
+    res["ir_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["ir"]["ms_per_step"]
+    if check and cpu_steps > 0:
+        # FP64 CPU run (the oracle, all host threads) of the first cpu_steps steps
+        from oracle import oracle as O
+        x = b.clone()
+        with P.Adi(P.ADI_FP64, dims, sets64, handle=h) as adi:
+            adi.step(x, cpu_steps)
+        with P.Adi(P.ADI_IR, dims, sets32, handle=h, max_refine=max_refine, tol=tol) as adi:
+            xi = b.clone()
+            adi.step(xi, cpu_steps)
+        torch.cuda.synchronize()
+        xc = b.cpu().numpy()
+        c64 = [[a.cpu().numpy() for a in cs] for cs in sets64]
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            for sd in range(3):
+                O.gpsv_batch(*c64[sd], xc, dims, sd)
+        cpu_s = time.perf_counter() - t0
+        del c64
+        res["fp64_max_rel_err_vs_cpu_fp64"] = max_rel(x.cpu().numpy(), xc)
+        c32w = [[a.double().cpu().numpy() for a in cs] for cs in sets32]
+        xw = b.cpu().numpy()
+        for _ in range(cpu_steps):
+            for sd in range(3):
+                O.gpsv_batch(*c32w[sd], xw, dims, sd)
+        res["ir_max_rel_err_vs_cpu_fp64"] = max_rel(xi.cpu().numpy(), xw)
+        res["cpu_check_steps"] = cpu_steps
+        res["cpu_fp64_oracle_grows"] = 3 * n * cpu_steps / cpu_s / 1e9
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +447,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-13)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other configurations (weak set, 256^3, varlen, hyperdiffusion)")
+    ap.add_argument("--c5-cpu-steps", type=int, default=2,
+                    help="hyperdiffusion steps re-run on the CPU oracle for the error check")
     ap.add_argument("--probe", action="store_true",
                     help="also launch the 6-read/1-write bandwidth probe once per precision "
                          "over the same field size (calibrates profiler byte counters)")
@@ -308,6 +576,24 @@ def main():
     if not args.no_cpu:
         log("bench: cpu baseline (reference CPU solver)")
         result["cpu_baseline"] = cpu_baseline(c64_np, b_np, dims)
+    del c64_np, b_np, c64, c32, b32, ds, dl, d, du, dw, b
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        chk = not args.no_check
+        log("bench: weak set (c = 1.05 sum|off|)")
+        result["weak_set"] = weak_set(P, torch, W, h, args.size, args.steps, args.max_refine,
+                                      args.tol, chk)
+        torch.cuda.empty_cache()
+        log("bench: 256^3, 50 iterations, FP64 / FP32 / shared-fact")
+        wl = {"c2_256cubed": workload_c2(P, torch, W, h, chk)}
+        torch.cuda.empty_cache()
+        log("bench: 8192 variable-length systems")
+        wl["c4_varlen"] = workload_c4(P, torch, W, h, chk)
+        torch.cuda.empty_cache()
+        log("bench: hyperdiffusion 384x256x640, 100 steps")
+        wl["c5_hyperdiffusion"] = workload_c5(P, torch, W, h, chk, cpu_steps=args.c5_cpu_steps,
+                                              max_refine=args.max_refine, tol=args.tol)
+        result["workloads"] = wl
     h.close()
     print(json.dumps(result), flush=True)
 

@@ -10,8 +10,14 @@ are this library's own, fixed so that tests and benchmarks are reproducible:
   along (the same five arrays serve x, y and z, reference apps/adi_cuda.cu:167-176).
   Draw order: ds, dl, du, dw, ratio, rhs.
 * ``weak``: as ``dominant`` with ratio fixed at 1.05 (weakly dominant set).
-* ``hyperdiffusion``: implicit fourth-order diffusion I + s*[1,-4,6,-4,1] with
-  s = nu*dt/h^4 per grid line, nu ~ U(0.5, 2).
+* ``hyperdiffusion``: implicit fourth-order diffusion, one operator per
+  direction: along every grid line of direction d the system is
+  I + s*[1,-4,6,-4,1] with s = nu*dt/h^4 and nu ~ U(0.5, 2) drawn once per
+  line (seed 7 by default; the initial field is N(0,1), seed 8).  The stencil
+  is truncated at the ends of a line (values beyond the boundary are zero,
+  i.e. the out-of-range couplings are zeroed), so each line's matrix is
+  symmetric positive definite with eigenvalues >= 1 and every direction
+  solve is a contraction in the 2-norm.  Default dt/h^4 = 0.25.
 * ``varlen_offsets``: lengths round(exp(U(ln lo, ln hi))) packed contiguously.
 
 Host generation uses numpy (``numpy.random.default_rng(seed)``); device
@@ -69,24 +75,77 @@ def dominant_torch(dims: Sequence[int], seed: int, ratio: Tuple[float, float] = 
     return tuple(a.to(dtype) for a in (ds, dl, d, du, dw, rhs))
 
 
-def hyperdiffusion_coeffs_np(dims: Sequence[int], seed: int, dt_over_h4: float,
-                             dtype=np.float64):
-    """Coefficients of I + s*D4 with s = nu*dt/h^4, nu ~ U(0.5, 2) per element.
+HYPERDIFFUSION_DT_OVER_H4 = 0.25
 
-    Rows: ds = dw = s, dl = du = -4s, d = 1 + 6s.  At the ends of a line the
-    out-of-range couplings are simply not part of the system (the stencil is
-    truncated: zero values beyond the boundary).  The same arrays serve all
-    three directions.
-    """
-    n = n_elems(dims)
+
+def _line_view(dims, d):
+    """numpy/torch shape that broadcasts a per-line value along axis d
+    (arrays are stored dims[0]-fastest, i.e. numpy order reversed)."""
+    nd = len(dims)
+    view = [int(v) for v in reversed(dims)]
+    view[nd - 1 - d] = 1
+    return view
+
+
+def hyperdiffusion_np(dims: Sequence[int], seed: int = 7,
+                      dt_over_h4: float = HYPERDIFFUSION_DT_OVER_H4, dtype=np.float64):
+    """Per-direction coefficient sets [(ds, dl, d, du, dw) for d in axes],
+    flat host arrays in the standard layout."""
     rng = np.random.default_rng(seed)
-    s = rng.uniform(0.5, 2.0, n) * dt_over_h4
-    ds = s.copy()
-    dw = s.copy()
-    dl = -4.0 * s
-    du = -4.0 * s
-    d = 1.0 + 6.0 * s
-    return tuple(a.astype(dtype) for a in (ds, dl, d, du, dw))
+    sets = []
+    nd = len(dims)
+    shape = tuple(int(v) for v in reversed(dims))
+    for ax in range(nd):
+        nu = rng.uniform(0.5, 2.0, _line_view(dims, ax))
+        s = np.broadcast_to(nu * dt_over_h4, shape)
+        i = np.arange(dims[ax]).reshape([-1 if q == nd - 1 - ax else 1 for q in range(nd)])
+        n = dims[ax]
+        ds = np.where(i >= 2, s, 0.0)
+        dl = np.where(i >= 1, -4.0 * s, 0.0)
+        dd = 1.0 + 6.0 * s
+        du = np.where(i + 1 < n, -4.0 * s, 0.0)
+        dw = np.where(i + 2 < n, s, 0.0)
+        sets.append(tuple(np.ascontiguousarray(np.broadcast_to(a, shape), dtype=dtype).reshape(-1)
+                          for a in (ds, dl, dd, du, dw)))
+    return sets
+
+
+def hyperdiffusion_torch(dims: Sequence[int], seed: int = 7,
+                         dt_over_h4: float = HYPERDIFFUSION_DT_OVER_H4, dtype=None,
+                         device="cuda"):
+    """hyperdiffusion_np's recipe generated on the GPU (torch generator)."""
+    import torch
+    dtype = dtype or torch.float64
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    nd = len(dims)
+    shape = tuple(int(v) for v in reversed(dims))
+    sets = []
+    for ax in range(nd):
+        nu = torch.empty(_line_view(dims, ax), dtype=torch.float64, device=device)
+        nu.uniform_(0.5, 2.0, generator=g)
+        s = (nu * dt_over_h4).expand(shape)
+        idx = [-1 if q == nd - 1 - ax else 1 for q in range(nd)]
+        i = torch.arange(dims[ax], device=device).reshape(idx)
+        n = dims[ax]
+        zero = torch.zeros((), dtype=torch.float64, device=device)
+        ds = torch.where(i >= 2, s, zero)
+        dl = torch.where(i >= 1, -4.0 * s, zero)
+        dd = 1.0 + 6.0 * s
+        du = torch.where(i + 1 < n, -4.0 * s, zero)
+        dw = torch.where(i + 2 < n, s, zero)
+        sets.append(tuple(a.expand(shape).contiguous().reshape(-1).to(dtype)
+                          for a in (ds, dl, dd, du, dw)))
+    return sets
+
+
+def gaussian_field_torch(dims: Sequence[int], seed: int = 8, dtype=None, device="cuda"):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    x = torch.empty(n_elems(dims), dtype=torch.float64, device=device)
+    x.normal_(0.0, 1.0, generator=g)
+    return x.to(dtype or torch.float64)
 
 
 def varlen_offsets(nsys: int, seed: int, lo: int = 32, hi: int = 8192) -> np.ndarray:
