@@ -86,23 +86,121 @@ __global__ void __launch_bounds__(MAXT)
   const int64_t b = ln.base(valid ? sys : ln.n_sys - 1) + (int64_t)p * M * st;
 
   T s[M], l[M], dd[M], u[M], w[M], f[M];
+  const int len = (int)ln.len;
+  if (P * M == len) {
 #pragma unroll
-  for (int i = 0; i < M; ++i) {
-    const int64_t e = b + i * st;
-    s[i] = ds[e];
-    l[i] = dl[e];
-    dd[i] = d[e];
-    u[i] = du[e];
-    w[i] = dw[e];
-    f[i] = x[e];
+    for (int i = 0; i < M; ++i) {
+      const int64_t e = b + i * st;
+      s[i] = ds[e];
+      l[i] = dl[e];
+      dd[i] = d[e];
+      u[i] = du[e];
+      w[i] = dw[e];
+      f[i] = x[e];
+    }
+  } else {
+    // padded length: rows past the end are identity rows, couplings of real
+    // rows past the end are zero
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      const int r = p * M + i;
+      const int64_t e = b + i * st;
+      const bool in = r < len;
+      s[i] = in ? ds[e] : T(0);
+      l[i] = in ? dl[e] : T(0);
+      dd[i] = in ? d[e] : T(1);
+      u[i] = (r + 1 < len) ? du[e] : T(0);
+      w[i] = (r + 2 < len) ? dw[e] : T(0);
+      f[i] = in ? x[e] : T(0);
+    }
   }
 
   LdsEx ex{smem_raw, sl, p, S, P};
   part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[b + i * st] = f[i];
+    for (int i = 0; i < M; ++i)
+      if (p * M + i < len) x[b + i * st] = f[i];
   }
+}
+
+// ---------------------------------------------------------------------------
+// x direction, any length ("transpose"): the S consecutive systems of a
+// workgroup are one contiguous span of S * len elements per array.  Each
+// array is staged through LDS with coalesced loads (consecutive threads,
+// consecutive elements, whatever the alignment), then every lane reads its M
+// rows out of LDS -- an in-tile transpose instead of a global one.  Rows past
+// a system's end (len <= L * M) are identity rows; the solution goes back
+// through the same tile.  HBM traffic is the compulsory 7 words per row.
+// ---------------------------------------------------------------------------
+template <typename T, int M>
+__global__ void __launch_bounds__(256)
+    k_tr_x(const T *__restrict__ ds, const T *__restrict__ dl, const T *__restrict__ d,
+           const T *__restrict__ du, const T *__restrict__ dw, T *__restrict__ x,
+           int64_t n_sys, int len, int lshift) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *tile = reinterpret_cast<T *>(smem_raw);
+  const int L = 1 << lshift, S = (int)blockDim.x >> lshift;
+  const int sl = (int)threadIdx.x >> lshift, p = (int)threadIdx.x & (L - 1);
+  const int64_t sys0 = (int64_t)blockIdx.x * S;
+  const int ns = (int)((n_sys - sys0) < S ? (n_sys - sys0) : S);
+  const int64_t base = sys0 * len;
+  const int span = ns * len;
+  const int own = sl * len; // this lane's system inside the span
+  const bool act = sl < ns;
+  auto slot = [&](int e) { return e + e / M; }; // one pad per M elements
+  auto stage = [&](const T *__restrict__ src) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < span; e += blockDim.x) tile[slot(e)] = src[base + e];
+    __syncthreads();
+  };
+  T s[M], l[M], dd[M], u[M], w[M], f[M];
+  stage(ds);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    s[i] = (act && r < len && r >= 2) ? tile[slot(own + r)] : T(0);
+  }
+  stage(dl);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    l[i] = (act && r < len && r >= 1) ? tile[slot(own + r)] : T(0);
+  }
+  stage(d);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    dd[i] = (act && r < len) ? tile[slot(own + r)] : T(1);
+  }
+  stage(du);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    u[i] = (act && r + 1 < len) ? tile[slot(own + r)] : T(0);
+  }
+  stage(dw);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    w[i] = (act && r + 2 < len) ? tile[slot(own + r)] : T(0);
+  }
+  stage(x);
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    f[i] = (act && r < len) ? tile[slot(own + r)] : T(0);
+  }
+  WaveEx ex{(int)(threadIdx.x & 63), p, L};
+  part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int r = p * M + i;
+    if (act && r < len) tile[slot(own + r)] = f[i];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < span; e += blockDim.x) x[base + e] = tile[slot(e)];
 }
 
 // ---------------------------------------------------------------------------
@@ -153,9 +251,9 @@ template <typename T, int M>
 hipError_t go_yz(const Diags<T> &a, T *x, const Lines &ln, int S,
                  hipStream_t st) {
   constexpr int MAXT = yz_maxt<T, M>();
-  const int P = (int)(ln.len / M);
+  const int P = (int)((ln.len + M - 1) / M);
   const int block = S * P;
-  if (block > MAXT || block % 64 != 0) return hipErrorInvalidConfiguration;
+  if (block > MAXT) return hipErrorInvalidConfiguration;
   const size_t smem = (size_t)10 * S * P * sizeof(T);
   const int64_t grid = (ln.n_sys + S - 1) / S;
   if (smem > 65536) {
@@ -205,11 +303,42 @@ hipError_t launch_part_x(const Diags<T> &a, T *x, const Lines &ln, int rows,
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len) {
   if (rows != 4 && rows != 6 && rows != 8 && rows != 12 && rows != 16)
     return false;
-  if (len % rows != 0) return false;
-  const int64_t P = len / rows;
-  const int64_t block = P * sys_per_wg;
-  return block <= yz_maxt_rt(elem_bytes, rows) && block % 64 == 0;
+  if (len < 3) return false;
+  const int64_t P = (len + rows - 1) / rows; // a partial last chunk is padded
+  const int64_t block = P * sys_per_wg;     // (a partial last wavefront is fine)
+  return block <= yz_maxt_rt(elem_bytes, rows);
 }
+
+// LDS-staged x solve: rows 8 or 16, lanes a power of two <= 64, any len with
+// lanes * rows >= len > (lanes / 2) * rows (no idle chunk)
+bool tr_x_supported(int rows, int lanes, int64_t len) {
+  if (rows != 8 && rows != 16) return false;
+  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) return false;
+  return len >= 3 && len <= (int64_t)rows * lanes;
+}
+
+template <typename T>
+hipError_t launch_tr_x(const Diags<T> &a, T *x, const Lines &ln, int rows, int lanes,
+                       hipStream_t st) {
+  if (!tr_x_supported(rows, lanes, ln.len) || ln.dir != 0) return hipErrorInvalidValue;
+  int lshift = 0;
+  while ((1 << lshift) < lanes) ++lshift;
+  const int block = 256, S = block >> lshift;
+  const int64_t grid = (ln.n_sys + S - 1) / S;
+  const int64_t span = (int64_t)S * ln.len;
+  const size_t smem = (size_t)(span + span / rows + 1) * sizeof(T);
+  if (rows == 8)
+    hipLaunchKernelGGL((k_tr_x<T, 8>), dim3((unsigned)grid), dim3(block), smem, st, a.ds,
+                       a.dl, a.d, a.du, a.dw, x, ln.n_sys, (int)ln.len, lshift);
+  else
+    hipLaunchKernelGGL((k_tr_x<T, 16>), dim3((unsigned)grid), dim3(block), smem, st, a.ds,
+                       a.dl, a.d, a.du, a.dw, x, ln.n_sys, (int)ln.len, lshift);
+  return hipGetLastError();
+}
+template hipError_t launch_tr_x<float>(const Diags<float> &, float *, const Lines &, int, int,
+                                       hipStream_t);
+template hipError_t launch_tr_x<double>(const Diags<double> &, double *, const Lines &, int,
+                                        int, hipStream_t);
 
 template <typename T>
 hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,

@@ -173,6 +173,23 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
     }
   }
 
+  // LDS-staged x solve for any length (the "transpose" capability): asked
+  // for by name, or the automatic choice when no partitioned shape fits
+  auto plan_tr = [&](Plan &q) {
+    for (int r : {8, 16}) {
+      int lanes = 1;
+      while ((int64_t)lanes * r < ln.len) lanes <<= 1;
+      if (tr_x_supported(r, lanes, ln.len)) {
+        q.kind = Plan::TrX;
+        q.rows = r;
+        q.lanes = lanes;
+        return true;
+      }
+    }
+    return false;
+  };
+  if (ln.dir == 0 && want == "transpose" && plan_tr(pl)) return pl;
+
   if (ln.dir == 0) {
     bool al = true;
     for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
@@ -183,6 +200,11 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
         pl.kind = Plan::PartX;
         pl.rows = tuned.rows;
         pl.lanes = tuned.lanes;
+        return pl;
+      }
+      if (have_tuned && tuned.kind == Plan::TrX &&
+          tr_x_supported(tuned.rows, tuned.lanes, ln.len)) {
+        pl = tuned;
         return pl;
       }
       // preferred rows per lane: 8 (FP64) / 16 (FP32), then the others
@@ -202,6 +224,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
         return pl;
       }
     }
+    if ((want == "auto" || want == "thomas-pcr") && plan_tr(pl)) return pl;
     pl.kind = Plan::Naive;
     return pl;
   }
@@ -218,8 +241,10 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
     static const int pref64[] = {8, 6, 12, 4, 16};
     static const int pref32[] = {16, 12, 8, 6, 4};
     const int *pref = is64 ? pref64 : pref32;
-    for (int k = 0; k < 5; ++k) {
-      const int r = pref[k];
+    // lengths that split into whole chunks first, then padded last chunks
+    for (int k = 0; k < 10; ++k) {
+      const int r = pref[k % 5];
+      if ((k < 5) != (ln.len % r == 0)) continue;
       for (int S = 64; S >= 16; S >>= 1) {
         if (part_yz_supported(elem_bytes, r, S, ln.len)) {
           pl.kind = Plan::PartYZ;
@@ -329,6 +354,9 @@ std::string plan_config(const Plan &pl) {
   case Plan::SfX:
     std::snprintf(buf, sizeof buf, "shared-fact rows=%d lanes=%d", pl.rows, pl.lanes);
     return buf;
+  case Plan::TrX:
+    std::snprintf(buf, sizeof buf, "transpose rows=%d lanes=%d", pl.rows, pl.lanes);
+    return buf;
   case Plan::SfYZ:
     std::snprintf(buf, sizeof buf, "shared-fact rows=%d sys_per_wg=%d", pl.rows,
                   pl.sys_per_wg);
@@ -348,6 +376,7 @@ hipError_t launch_plan(const Plan &pl, const Diags<T> &a, T *x, const Lines &ln,
   case Plan::PartX: return launch_part_x<T>(a, x, ln, pl.rows, pl.lanes, st);
   case Plan::PartYZ:
     return launch_part_yz<T>(a, x, ln, pl.rows, pl.sys_per_wg, st);
+  case Plan::TrX: return launch_tr_x<T>(a, x, ln, pl.rows, pl.lanes, st);
   case Plan::Naive:
     if (buffer == nullptr) return hipErrorInvalidValue;
     return launch_thomas_lane<T>(a, x, ln, static_cast<T *>(buffer), st);
@@ -449,7 +478,9 @@ pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
     return PENTAD_STATUS_EXECUTION_FAILED;
   }
   note_kernel(c, ln.dir,
-              pl.kind == Plan::Naive ? "naive" : (sf ? "shared-fact" : "thomas-pcr"),
+              pl.kind == Plan::Naive
+                  ? "naive"
+                  : (sf ? "shared-fact" : (pl.kind == Plan::TrX ? "transpose" : "thomas-pcr")),
               plan_config(pl));
   return PENTAD_STATUS_SUCCESS;
 }

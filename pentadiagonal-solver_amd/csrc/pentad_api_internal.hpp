@@ -14,7 +14,8 @@ namespace pentad {
 // Which kernel a solve launches and with what shape parameters.
 struct Plan {
   enum Kind {
-    Naive = 0, PartX = 1, PartYZ = 2, IrLane = 3, IrX = 4, IrYZ = 5, SfX = 6, SfYZ = 7
+    Naive = 0, PartX = 1, PartYZ = 2, IrLane = 3, IrX = 4, IrYZ = 5, SfX = 6, SfYZ = 7,
+    TrX = 8
   } kind = Naive;
   int rows = 0;       // rows per lane (partitioned kernels)
   int lanes = 0;      // lanes per system (x)

@@ -101,6 +101,13 @@ hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
                           int sys_per_wg, hipStream_t st);
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
 
+// LDS-staged x solve for any length (rows 8 or 16, lanes a power of two,
+// len <= rows * lanes; rows past the end are identity rows): "transpose".
+bool tr_x_supported(int rows, int lanes, int64_t len);
+template <typename T>
+hipError_t launch_tr_x(const Diags<T> &a, T *x, const Lines &ln, int rows, int lanes,
+                       hipStream_t st);
+
 // Variable-length batch: systems packed back to back (offsets), solved in
 // one persistent launch over work items of S systems x P chunks of
 // VL_ROWS rows (S * P = VL_BLOCK); lengths 3 .. VL_BLOCK * VL_ROWS.

@@ -48,7 +48,9 @@ std::vector<Plan> candidates(const Lines &ln, int elem_bytes, bool ir,
   const int nrows = ir ? 4 : 6;
   for (int k = 0; k < nrows; ++k) {
     const int r = rows[k];
-    if (ln.len % r) continue;
+    // the plain strided kernels pad a partial last chunk; the others need
+    // whole chunks
+    if (ln.len % r && (ir || ln.dir == 0)) continue;
     if (ln.dir == 0) {
       if (!aligned) continue;
       const int64_t lanes = ln.len / r;
@@ -72,6 +74,19 @@ std::vector<Plan> candidates(const Lines &ln, int elem_bytes, bool ir,
           p.sys_per_wg = S;
           out.push_back(p);
         }
+      }
+    }
+  }
+  if (!ir && ln.dir == 0) { // LDS-staged solve ("transpose"), any length
+    for (int r : {8, 16}) {
+      int lanes = 1;
+      while ((int64_t)lanes * r < ln.len) lanes <<= 1;
+      if (tr_x_supported(r, lanes, ln.len)) {
+        Plan p;
+        p.kind = Plan::TrX;
+        p.rows = r;
+        p.lanes = lanes;
+        out.push_back(p);
       }
     }
   }

@@ -74,9 +74,12 @@ def test_host_only_calls_without_gpu():
     # register-resident kernels need none
     assert lib.pentadsolver_D_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims, 3, 1) == 0
     assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims, 3, 2) == 0
-    # odd length: falls back to the scratch kernel
+    # odd length: the partitioned kernel pads its last chunk, no scratch
     dims2 = (ctypes.c_int * 2)(8, 37)
-    assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims2, 2, 1) == 3 * 8 * 37 * 4
+    assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims2, 2, 1) == 0
+    # a length past every on-chip kernel falls back to the scratch kernel
+    dims3 = (ctypes.c_int * 2)(3000, 8)
+    assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims3, 2, 0) == 3 * 8 * 3000 * 4
     assert lib.pentadsolver_kernel_that_ran(5) == b"none"
     lib.pentadsolver_destroy(ctypes.byref(h))
     assert not h.value

@@ -182,12 +182,50 @@ def test_buffer_contract_and_status():
     dims = [37, 8, 8]
     arrs = gen(dims, 1)
     t = [dev(a) for a in arrs]
-    need = P.gpsv_batch_buffer_size_ext(*t, dims, 0)
-    assert need == 3 * 37 * 64 * 8
-    with pytest.raises(P.PentadError):
-        P.gpsv_batch(*t, dims, 0, buffer=None)
-    with pytest.raises(P.PentadError):
-        P.gpsv_batch(*t, [37, 8, 8], 3)
+    with P.Handle() as h:
+        assert P.gpsv_batch_buffer_size_ext(*t, dims, 0, h) == 0  # LDS-staged kernel
+        h.set_algo(0, "naive")
+        need = P.gpsv_batch_buffer_size_ext(*t, dims, 0, h)
+        assert need == 3 * 37 * 64 * 8
+        with pytest.raises(P.PentadError):
+            P.gpsv_batch(*t, dims, 0, buffer=None, handle=h)
+        with pytest.raises(P.PentadError):
+            P.gpsv_batch(*t, [37, 8, 8], 3, handle=h)
+
+
+@pytest.mark.parametrize("n", [3, 5, 7, 37, 100, 300, 511, 700, 1000])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_transpose_x_any_length(n, dtype):
+    """x lengths with no whole-chunk partition: the LDS-staged kernel
+    (rows past the end are identity rows) is the automatic choice."""
+    dims = [n, 13, 7]
+    print(f"transpose x len={n} {np.dtype(dtype).name}", flush=True)
+    with P.Handle() as h:
+        check(gen(dims, n, dtype), dims, 0, h)
+        assert h.kernel_config(0).startswith("transpose"), h.kernel_config(0)
+        assert P.kernel_that_ran(0) == "transpose"
+
+
+@pytest.mark.parametrize("n", [64, 512, 1024])
+def test_transpose_x_forced(n):
+    dims = [n, 20, 3]
+    with P.Handle() as h:
+        h.set_algo(0, "transpose")
+        check(gen(dims, 3 * n, np.float64), dims, 0, h)
+        assert h.kernel_config(0).startswith("transpose")
+
+
+@pytest.mark.parametrize("n", [5, 37, 100, 200, 300])
+@pytest.mark.parametrize("sd", [1, 2])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_partition_yz_padded_lengths(n, sd, dtype):
+    """y/z lengths that do not split into whole chunks: the partitioned
+    kernel pads the last chunk with identity rows."""
+    dims = [40, n, 3] if sd == 1 else [40, 3, n]
+    print(f"yz padded len={n} sd={sd} {np.dtype(dtype).name}", flush=True)
+    with P.Handle() as h:
+        check(gen(dims, n + sd, dtype), dims, sd, h)
+        assert h.kernel_config(sd).startswith("thomas-pcr"), h.kernel_config(sd)
 
 
 def test_streams_and_handles():
