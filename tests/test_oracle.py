@@ -264,3 +264,31 @@ def test_oracle_mutations(mut):
     x = _thomas_py(fx["ds"], fx["dl"], fx["d"], fx["du"], fx["dw"], fx["x"], mut)
     ok = O.allclose_reference_rule(fx["u"], x, np.float64)
     assert ok == (mut is None)
+
+
+def test_oracle_varlen_vs_lapack_and_reference():
+    """Packed variable-length systems (lengths 3..8192): every system against
+    LAPACK's banded LU; systems up to the reference CPU solver's N_MAX (1024,
+    src/cpu/singlenode/pentad_cpu.cpp:11) also against the reference build,
+    one 1-D solve each.  Lengths above 1024 have no reference counterpart and
+    are pinned by LAPACK only."""
+    rng = np.random.default_rng(31)
+    lens = np.concatenate([[3, 4, 5, 1024, 1025, 8192],
+                           np.rint(np.exp(rng.uniform(np.log(3), np.log(8192), 30)))]).astype(np.int64)
+    off = np.zeros(len(lens) + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    n = int(off[-1])
+    ds, dl, du, dw = (rng.uniform(-1, 1, n) for _ in range(4))
+    d = rng.uniform(1.2, 3.0, n) * (abs(ds) + abs(dl) + abs(du) + abs(dw))
+    b = rng.uniform(-1, 1, n)
+    x = O.varlen(ds, dl, d, du, dw, b, off)
+    for k in range(len(lens)):
+        lo, hi = int(off[k]), int(off[k + 1])
+        sl = slice(lo, hi)
+        ref = scipy_solve(ds[sl], dl[sl], d[sl], du[sl], dw[sl], b[sl], [hi - lo], 0)
+        assert relerr(x[sl], ref) < 1e-13, (k, hi - lo)
+        if O.ref_available("avx") and hi - lo <= 1024:
+            xr = b[sl].copy()
+            O.ref_gpsv_batch(ds[sl].copy(), dl[sl].copy(), d[sl].copy(), du[sl].copy(),
+                             dw[sl].copy(), xr, [hi - lo], 0)
+            assert relerr(x[sl], xr) < 1e-13, (k, hi - lo)
