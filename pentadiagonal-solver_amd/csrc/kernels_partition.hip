@@ -30,39 +30,51 @@ namespace pentad {
 //              systems (whose runs share 128-byte lines) use one L2.
 // (No occupancy hint: forcing FP64 rows=8 down to 128 VGPRs for 4 waves per
 // SIMD measured 8% slower than the compiler's 138 VGPRs at 3 waves.)
-template <typename T, int M, bool ROT>
+// Q (ROT only): the workgroup solves Q groups of blockDim / L systems one
+// after the other into one tile, so that each row of the tile is a run of
+// Q * blockDim / L systems (wider runs, fewer partial 128-byte lines).
+template <typename T, int M, bool ROT, int Q>
 __global__ void __launch_bounds__(512)
     k_part_x(const T *__restrict__ ds, const T *__restrict__ dl,
              const T *__restrict__ d, const T *__restrict__ du,
              const T *__restrict__ dw, const T *b, T *out, int64_t n_sys,
              int64_t len, int lshift /* log2 lanes per system */, XMap map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t blk = ROT ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-  const int64_t g = blk * blockDim.x + threadIdx.x;
   const int L = 1 << lshift;
-  const int64_t sys = g >> lshift;
-  const int p = (int)(g & (L - 1));
-  const bool valid = sys < n_sys;
-  const int64_t sc = valid ? sys : n_sys - 1;
-  const int64_t slo = map.lo(sc), shi = map.hi(sc);
-  const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
-  const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
-
-  T s[M], l[M], dd[M], u[M], w[M], f[M];
-  RowsIO<T, M>::load(ds + offc, s);
-  RowsIO<T, M>::load(dl + offc, l);
-  RowsIO<T, M>::load(d + offc, dd);
-  RowsIO<T, M>::load(du + offc, u);
-  RowsIO<T, M>::load(dw + offc, w);
-  RowsIO<T, M>::load(b + off, f);
-
+  const int s0shift = __builtin_ctz(blockDim.x) - lshift; // systems per pass
+  const int sl = (int)threadIdx.x >> lshift;
+  const int p = (int)threadIdx.x & (L - 1);
   WaveEx ex{(int)(threadIdx.x & 63), p, L};
-  part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+#pragma unroll 1
+  for (int q = 0; q < Q; ++q) {
+    const int64_t sys = ((blk * Q + q) << s0shift) + sl;
+    const bool valid = sys < n_sys;
+    const int64_t sc = valid ? sys : n_sys - 1;
+    const int64_t slo = map.lo(sc), shi = map.hi(sc);
+    const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+    const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
+
+    T s[M], l[M], dd[M], u[M], w[M], f[M];
+    RowsIO<T, M>::load(ds + offc, s);
+    RowsIO<T, M>::load(dl + offc, l);
+    RowsIO<T, M>::load(d + offc, dd);
+    RowsIO<T, M>::load(du + offc, u);
+    RowsIO<T, M>::load(dw + offc, w);
+    RowsIO<T, M>::load(b + off, f);
+
+    part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+    if constexpr (ROT) {
+      tile_put<T, M>(reinterpret_cast<T *>(smem_raw), f, p, Q << s0shift, (q << s0shift) + sl);
+    } else {
+      if (valid) RowsIO<T, M>::store(out + off, f);
+    }
+  }
   if constexpr (ROT) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    stage_rotated_store<T, M>(reinterpret_cast<T *>(smem_raw), f, p, lshift, (int)len, out,
-                              n_sys, blk, map);
-  } else {
-    if (valid) RowsIO<T, M>::store(out + off, f);
+    __syncthreads();
+    const int sshift = s0shift + __builtin_ctz(Q);
+    tile_flush<T, M>(reinterpret_cast<const T *>(smem_raw), sshift, (int)len, out, n_sys,
+                     blk << sshift, map);
   }
 }
 
@@ -213,25 +225,31 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                 int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
-  // FP32 rotated stores: 512-thread workgroups own twice the systems, so each
-  // row of the tile is a 64-byte run (measured ~12% faster than 256 at 512^3;
-  // FP64 is faster at 256, where the state allows 3 waves per SIMD)
-  const int block = (rot && sizeof(T) == 4) ? 512 : 256;
-  const int64_t threads = n_sys * lanes;
-  const int64_t grid = (threads + block - 1) / block;
-  if (rot) {
-    const size_t smem = rotated_tile_bytes<T>(block, M, len);
-    if (smem > 65536) {
-      const hipError_t e = hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&k_part_x<T, M, true>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_part_x<T, M, true>), dim3((unsigned)grid), dim3(block), smem,
+  if (!rot) {
+    const int block = 256;
+    const int64_t grid = (n_sys * lanes + block - 1) / block;
+    hipLaunchKernelGGL((k_part_x<T, M, false, 1>), dim3((unsigned)grid), dim3(block), 0,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
-  } else
-    hipLaunchKernelGGL((k_part_x<T, M, false>), dim3((unsigned)grid), dim3(block), 0,
-                       st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+    return hipGetLastError();
+  }
+  // rotated stores: FP32 in 512-thread workgroups, FP64 in 256 (3 waves per
+  // SIMD fit its state); Q passes per workgroup widen the tile to 128-byte
+  // runs at 512 lines: FP64 rows=8 x 64 lanes -> 16 systems (1.64 ms per
+  // 512^3 direction against 1.76 for Q = 2 and 1.89 for Q = 1; Q = 8 is
+  // 1.73), FP32 rows=16 x 32 lanes -> 32 systems (Q = 2)
+  const int block = sizeof(T) == 4 ? 512 : 256;
+  constexpr int Q = sizeof(T) == 4 ? 2 : 4;
+  const int64_t per_wg = (int64_t)(block >> lshift) * Q;
+  const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+  const size_t smem = rotated_tile_bytes<T>(block * Q, M, len);
+  if (smem > 65536) {
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_part_x<T, M, true, Q>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_part_x<T, M, true, Q>), dim3((unsigned)grid), dim3(block), smem, st,
+                     a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
   return hipGetLastError();
 }
 

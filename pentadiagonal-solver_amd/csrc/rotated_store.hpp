@@ -9,14 +9,13 @@
 // row: S consecutive elements per row, so every global store instruction
 // covers whole runs instead of one scattered element per lane.
 //
-// In the write-out each thread keeps one system (or V adjacent systems) for
-// every row it writes, so the system's output base -- a division by the
-// mapped extent -- is computed once per thread.  When the S systems are
-// adjacent in the output (unit system stride, no wrap inside the
-// workgroup, 16-byte aligned rows), each thread stores V = 16 / sizeof(T)
-// systems of a row with one 16-byte store.  Workgroups with adjacent
-// systems are dealt to the same XCD (xcd_tile), so the partial 128-byte
-// lines they share meet in one L2.
+// In the write-out each thread keeps one system for every row it writes, so
+// the system's output base -- a division by the mapped extent -- is computed
+// once per thread; a store instruction covers 64 / S rows of S consecutive
+// elements.  (16-byte stores of two or four systems per thread measured no
+// faster and wrote more partial lines.)  Workgroups with adjacent systems are
+// dealt to the same XCD (xcd_tile), so the partial 128-byte lines they share
+// meet in one L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,7 +23,6 @@
 #include <cstdint>
 
 #include "pentad_internal.hpp"
-#include "rows_io.hpp"
 
 namespace pentad {
 
@@ -34,45 +32,52 @@ __host__ __device__ constexpr size_t rotated_tile_bytes(int block, int M, int64_
   return (size_t)sizeof(T) * ((size_t)(block) / (size_t)(len / M) * (size_t)len + (size_t)(len / M));
 }
 
+// column col (0 <= col < S) of a tile of S systems: this lane's M rows
+template <typename T, int M>
+__device__ __forceinline__ void tile_put(T *tile, const T (&v)[M], int p, int S, int col) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) tile[(p * M + i) * S + col + p] = v[i];
+}
+
+// write a complete tile of S = 1 << sshift systems (sys0 ...) row by row
+template <typename T, int M>
+__device__ __forceinline__ void tile_flush(const T *tile, int sshift, int len, T *out,
+                                           int64_t n_sys, int64_t sys0, const XMap &map) {
+  const int S = 1 << sshift;
+  const int bshift = __builtin_ctz(blockDim.x);
+  if (sshift > bshift) {
+    // more systems than threads: each thread writes whole systems, the
+    // threads of one instruction covering consecutive systems of a row
+    for (int sb = 0; sb < S; sb += (int)blockDim.x) {
+      const int s = sb + (int)threadIdx.x;
+      const int64_t sys = sys0 + s;
+      if (sys >= n_sys) continue;
+      T *o = out + map.lo(sys) * map.out_lo + map.hi(sys) * map.out_hi;
+      for (int row = 0; row < len; ++row)
+        o[(int64_t)row * map.out_row] = tile[row * S + s + row / M];
+    }
+    return;
+  }
+  const int s = (int)threadIdx.x & (S - 1);
+  const int64_t sys = sys0 + s;
+  if (sys >= n_sys) return;
+  T *o = out + map.lo(sys) * map.out_lo + map.hi(sys) * map.out_hi;
+  const int rstep = (int)blockDim.x >> sshift; // >= 1 here
+  for (int row = (int)threadIdx.x >> sshift; row < len; row += rstep)
+    o[(int64_t)row * map.out_row] = tile[row * S + s + row / M];
+}
+
 template <typename T, int M>
 __device__ __forceinline__ void stage_rotated_store(T *tile, const T (&v)[M], int p,
                                                     int lshift, int len, T *out,
                                                     int64_t n_sys, int64_t blk,
                                                     const XMap &map) {
   const int sshift = __builtin_ctz(blockDim.x) - lshift;
-  const int S = 1 << sshift;                      // systems in this workgroup
   const int sl = (int)threadIdx.x >> lshift;      // this lane's system
   __syncthreads();                                // LDS may still hold other data
-#pragma unroll
-  for (int i = 0; i < M; ++i) tile[(p * M + i) * S + sl + p] = v[i];
+  tile_put<T, M>(tile, v, p, 1 << sshift, sl);
   __syncthreads();
-  const int64_t sys0 = blk * S;
-  constexpr int V = 16 / (int)sizeof(T);
-  const bool vec = S >= V && map.out_lo == 1 && map.S1 % S == 0 &&
-                   map.out_row % V == 0 && map.out_hi % V == 0 &&
-                   ((uintptr_t)out & 15) == 0 && sys0 + S <= n_sys;
-  if (vec) {
-    const int gshift = sshift - __builtin_ctz(V); // thread groups of V systems
-    const int s = ((int)threadIdx.x & ((1 << gshift) - 1)) * V;
-    const int64_t sys = sys0 + s;
-    T *o = out + map.lo(sys) + map.hi(sys) * map.out_hi;
-    const int rstep = (int)blockDim.x >> gshift;
-    for (int row = (int)threadIdx.x >> gshift; row < len; row += rstep) {
-      const T *t = tile + row * S + s + row / M;
-      T w[V];
-#pragma unroll
-      for (int k = 0; k < V; ++k) w[k] = t[k];
-      RowsIO<T, V>::store(o + (int64_t)row * map.out_row, w);
-    }
-  } else {
-    const int s = (int)threadIdx.x & (S - 1);
-    const int64_t sys = sys0 + s;
-    if (sys >= n_sys) return;
-    T *o = out + map.lo(sys) * map.out_lo + map.hi(sys) * map.out_hi;
-    const int rstep = (int)blockDim.x >> sshift;
-    for (int row = (int)threadIdx.x >> sshift; row < len; row += rstep)
-      o[(int64_t)row * map.out_row] = tile[row * S + s + row / M];
-  }
+  tile_flush<T, M>(tile, sshift, len, out, n_sys, blk << sshift, map);
 }
 
 } // namespace pentad
