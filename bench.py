@@ -51,6 +51,15 @@ sys.path.insert(0, ROOT)
 METRIC = "GRows/s + % HBM roofline, FP32+refinement vs plain FP64 at 1e-12 rel err"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ROW = {"ir": 36, "fp64": 56, "fp32": 28}
+# HBM bytes per launch measured with rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, in
+# KiB per dispatch; the factor 2 calibrated on the 6-read/1-write probe of
+# known size), keyed by (kernel configuration, line length).  Kept under
+# profiles/ with the commands that produced them.
+MEASURED_TRAFFIC_BYTES = {
+    ("ir thomas-pcr rows=8 lanes=64", 512): (3_690_774 + 1_165_366) * 1024,
+    ("thomas-pcr rows=8 lanes=64", 512): (7_164_428 + 1_048_576) * 1024,
+    ("thomas-pcr rows=16 lanes=32", 512): (5_626_725 + 524_288) * 1024,
+}
 
 
 def import_pkg():
@@ -548,7 +557,9 @@ def main():
                    "grid": dims, "mode": "FP32 + FP64 refinement"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": MEASURED_TRAFFIC_BYTES.get((head["kernels"][dom], dims[dom])),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE)",
+                     "algorithmic_bytes": head["bytes_per_row"] * n,
                      "kernel": f"direction {dom}: {head['kernels'][dom]}",
                      "bytes_per_row": head["bytes_per_row"],
                      "rows_per_launch": n, "kernel_ms": head["dir_ms"][dom]},
