@@ -240,6 +240,12 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
                        a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol,
                        steps, map);
   } else {
+    if (smem_a > 65536) {
+      const hipError_t e = hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&k_ir_x<M, R, false>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_a);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_ir_x<M, R, false>), dim3((unsigned)grid), dim3(block), smem_a,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps,
                        tol, steps, map);
