@@ -233,24 +233,30 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
     return hipGetLastError();
   }
   // rotated stores: FP32 in 512-thread workgroups, FP64 in 256 (3 waves per
-  // SIMD fit its state); Q passes per workgroup widen the tile to 128-byte
-  // runs at 512 lines: FP64 rows=8 x 64 lanes -> 16 systems (1.64 ms per
-  // 512^3 direction against 1.76 for Q = 2 and 1.89 for Q = 1; Q = 8 is
-  // 1.73), FP32 rows=16 x 32 lanes -> 32 systems (Q = 2)
+  // SIMD fit its state); Q passes per workgroup widen each tile row to a
+  // 128-byte run (measured at 512^3, FP64 rows=8 x 64 lanes: 1.89 / 1.76 /
+  // 1.64 / 1.73 ms for Q = 1 / 2 / 4 / 8); runs already 128 bytes wide keep
+  // Q = 1 (more workgroups)
   const int block = sizeof(T) == 4 ? 512 : 256;
-  constexpr int Q = sizeof(T) == 4 ? 2 : 4;
+  const int run = (block >> lshift) * (int)sizeof(T); // bytes per pass
+  const int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
   const int64_t per_wg = (int64_t)(block >> lshift) * Q;
   const int64_t grid = (n_sys + per_wg - 1) / per_wg;
   const size_t smem = rotated_tile_bytes<T>(block * Q, M, len);
-  if (smem > 65536) {
-    const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&k_part_x<T, M, true, Q>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((k_part_x<T, M, true, Q>), dim3((unsigned)grid), dim3(block), smem, st,
-                     a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
-  return hipGetLastError();
+  auto go = [&](auto kernel) {
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), smem, st, a.ds, a.dl, a.d,
+                       a.du, a.dw, b, out, n_sys, len, lshift, map);
+    return hipGetLastError();
+  };
+  if (Q == 1) return go(k_part_x<T, M, true, 1>);
+  if (Q == 2) return go(k_part_x<T, M, true, 2>);
+  return go(k_part_x<T, M, true, 4>);
 }
 
 // Largest workgroup each (type, M) instantiation is compiled for.  With W
