@@ -18,6 +18,7 @@
 #include "partition_core.hpp"
 #include "partition_exchange.hpp"
 #include "pentad_internal.hpp"
+#include "rows_io.hpp"
 
 namespace pentad {
 
@@ -72,13 +73,43 @@ __device__ __forceinline__ Item item_at(const int2 *items, int it) {
   return Item{v.x, v.y & 0xFFFF, v.y >> 16};
 }
 
-// rows [r0, r0 + M) of a system of length len starting at element off
-template <typename T, typename C, int M>
+// a chunk starting at element e of an array of C can use 16-byte vectors
+template <typename C, int M>
+__device__ __forceinline__ bool chunk_vec(int64_t e) {
+  return RowsIO<C, M>::VB == 16 && ((e * (int64_t)sizeof(C)) & 15) == 0;
+}
+
+// rows [r0, r0 + M) of a system of length len starting at element off;
+// VEC: 16-byte loads for whole aligned chunks (the refinement kernel, whose
+// registers are full, keeps the scalar path: its vector path spills more)
+template <typename T, typename C, int M, bool VEC>
 __device__ __forceinline__ void load_rows(const C *__restrict__ ds, const C *__restrict__ dl,
                                           const C *__restrict__ d, const C *__restrict__ du,
                                           const C *__restrict__ dw, int64_t off, int64_t r0,
                                           int64_t len, T (&s)[M], T (&l)[M], T (&dd)[M],
                                           T (&u)[M], T (&w)[M]) {
+  if (VEC && chunk_vec<C, M>(off + r0) && r0 + M <= len) {
+    // whole chunk inside the system, 16-byte aligned: vector loads, then
+    // zero the couplings that reach past either end
+    C t[M];
+    const int64_t e = off + r0;
+    RowsIO<C, M>::load(ds + e, t);
+#pragma unroll
+    for (int i = 0; i < M; ++i) s[i] = (r0 + i >= 2) ? (T)t[i] : T(0);
+    RowsIO<C, M>::load(dl + e, t);
+#pragma unroll
+    for (int i = 0; i < M; ++i) l[i] = (r0 + i >= 1) ? (T)t[i] : T(0);
+    RowsIO<C, M>::load(d + e, t);
+#pragma unroll
+    for (int i = 0; i < M; ++i) dd[i] = (T)t[i];
+    RowsIO<C, M>::load(du + e, t);
+#pragma unroll
+    for (int i = 0; i < M; ++i) u[i] = (r0 + i + 1 < len) ? (T)t[i] : T(0);
+    RowsIO<C, M>::load(dw + e, t);
+#pragma unroll
+    for (int i = 0; i < M; ++i) w[i] = (r0 + i + 2 < len) ? (T)t[i] : T(0);
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < M; ++i) {
     const int64_t r = r0 + i, e = off + r;
@@ -114,9 +145,14 @@ __global__ void __launch_bounds__(VL_BLOCK)
     }
     const int64_t r0 = (int64_t)p * M;
     T s[M], l[M], dd[M], u[M], w[M], f[M];
-    load_rows<T, T, M>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
+    load_rows<T, T, M, true>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
+    const bool whole = r0 + M <= len && chunk_vec<T, M>(off + r0);
+    if (whole) {
+      RowsIO<T, M>::load(x + off + r0, f);
+    } else {
 #pragma unroll
-    for (int i = 0; i < M; ++i) f[i] = (r0 + i < len) ? x[off + r0 + i] : T(0);
+      for (int i = 0; i < M; ++i) f[i] = (r0 + i < len) ? x[off + r0 + i] : T(0);
+    }
     if (P <= 64) {
       WaveEx ex{t & 63, p, P};
       part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
@@ -124,9 +160,13 @@ __global__ void __launch_bounds__(VL_BLOCK)
       VarEx ex{smem_raw, t, p, P, VL_BLOCK};
       part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
     }
+    if (whole) {
+      RowsIO<T, M>::store(x + off + r0, f);
+    } else {
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-      if (r0 + i < len) x[off + r0 + i] = f[i];
+      for (int i = 0; i < M; ++i)
+        if (r0 + i < len) x[off + r0 + i] = f[i];
+    }
   }
 }
 
@@ -154,7 +194,7 @@ __global__ void __launch_bounds__(VL_BLOCK)
     }
     const int64_t r0 = (int64_t)p * M;
     float s[M], l[M], dd[M], u[M], w[M];
-    load_rows<float, float, M>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
+    load_rows<float, float, M, false>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
     const int64_t nv = len - r0;
     const VarRhs b{x + off + r0, nv <= 0 ? 0 : (nv >= M ? M : (int)nv)};
     const RegCoef<M> A{s, l, dd, u, w};
