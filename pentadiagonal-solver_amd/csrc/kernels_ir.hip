@@ -96,15 +96,32 @@ __global__ void __launch_bounds__(MAXT)
   const int64_t b0 = ln.base(valid ? sys : ln.n_sys - 1) + (int64_t)p * M * st;
   float s[M], l[M], dd[M], u[M], w[M];
   double b[M], xs[M];
+  const int len = (int)ln.len;
+  if (P * M == len) {
 #pragma unroll
-  for (int i = 0; i < M; ++i) {
-    const int64_t e = b0 + i * st;
-    s[i] = ds[e];
-    l[i] = dl[e];
-    dd[i] = d[e];
-    u[i] = du[e];
-    w[i] = dw[e];
-    b[i] = x[e];
+    for (int i = 0; i < M; ++i) {
+      const int64_t e = b0 + i * st;
+      s[i] = ds[e];
+      l[i] = dl[e];
+      dd[i] = d[e];
+      u[i] = du[e];
+      w[i] = dw[e];
+      b[i] = x[e];
+    }
+  } else {
+    // padded length: identity rows past the end, couplings past it zero
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      const int r = p * M + i;
+      const int64_t e = b0 + i * st;
+      const bool in = r < len;
+      s[i] = in ? ds[e] : 0.0f;
+      l[i] = in ? dl[e] : 0.0f;
+      dd[i] = in ? d[e] : 1.0f;
+      u[i] = (r + 1 < len) ? du[e] : 0.0f;
+      w[i] = (r + 2 < len) ? dw[e] : 0.0f;
+      b[i] = in ? x[e] : 0.0;
+    }
   }
   ir_zero_out_of_range<M>(0, s, p == 0, p == P - 1);
   ir_zero_out_of_range<M>(1, l, p == 0, p == P - 1);
@@ -116,7 +133,8 @@ __global__ void __launch_bounds__(MAXT)
   const int k = ir_solve<M, R>(A, bv, xs, p, P, ex, max_steps, tol);
   if (valid) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[b0 + i * st] = xs[i];
+    for (int i = 0; i < M; ++i)
+      if (p * M + i < len) x[b0 + i * st] = xs[i];
   }
   if (t == 0) note_steps(steps_out, k);
 }
@@ -260,9 +278,9 @@ constexpr int IR_YZ_MAXT = 512;
 template <int M, int R>
 hipError_t go_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int S,
                     int max_steps, double tol, int *steps, hipStream_t st) {
-  const int P = (int)(ln.len / M);
+  const int P = (int)((ln.len + M - 1) / M); // a partial last chunk is padded
   const int block = S * P;
-  if (block > IR_YZ_MAXT || block % 64 != 0) return hipErrorInvalidConfiguration;
+  if (block > IR_YZ_MAXT) return hipErrorInvalidConfiguration;
   const size_t smem = (size_t)8 * S * P * sizeof(float) > (size_t)4 * S * P * sizeof(double)
                           ? (size_t)8 * S * P * sizeof(float)
                           : (size_t)4 * S * P * sizeof(double);
@@ -284,11 +302,11 @@ bool ir_x_supported(int rows, int64_t len) {
 
 bool ir_yz_supported(int rows, int sys_per_wg, int64_t len) {
   if (rows != 4 && rows != 6 && rows != 8 && rows != 10) return false;
-  if (len % rows) return false;
-  const int64_t P = len / rows;
+  if (len < 3) return false;
+  const int64_t P = (len + rows - 1) / rows; // a partial last chunk is padded
   if (P < 1 || P > 128) return false;
   const int64_t block = P * sys_per_wg;
-  return block <= IR_YZ_MAXT && block % 64 == 0;
+  return block <= IR_YZ_MAXT;
 }
 
 hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
@@ -318,7 +336,7 @@ hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int r
                         int sys_per_wg, int max_steps, double tol, int *steps,
                         hipStream_t st) {
   if (!ir_yz_supported(rows, sys_per_wg, ln.len)) return hipErrorInvalidValue;
-  const int R = ceil_log2(ln.len / rows);
+  const int R = ceil_log2((ln.len + rows - 1) / rows);
 #define IRYZ(MM, RR) \
   if (rows == MM && R == RR) return go_ir_yz<MM, RR>(a, x, ln, sys_per_wg, max_steps, tol, steps, st);
   IRYZ(4, 0) IRYZ(4, 1) IRYZ(4, 2) IRYZ(4, 3) IRYZ(4, 4) IRYZ(4, 5) IRYZ(4, 6) IRYZ(4, 7)

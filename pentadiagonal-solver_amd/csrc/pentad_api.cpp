@@ -308,7 +308,9 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
     return pl;
   }
   static const int pref[] = {8, 6, 4, 10};
-  for (int r : pref) {
+  for (int k = 0; k < 8; ++k) { // whole chunks first, then a padded last chunk
+    const int r = pref[k % 4];
+    if ((k < 4) != (ln.len % r == 0)) continue;
     for (int S = 64; S >= 4; S >>= 1) {
       if (ir_yz_supported(r, S, ln.len)) {
         pl.kind = Plan::IrYZ;

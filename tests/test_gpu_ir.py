@@ -123,3 +123,18 @@ def test_ir_lane_fallback_forced():
             x, steps = run_ir(a32, b, dims, sd, handle=h)
             assert h.kernel_config(sd) == "ir naive"
             assert relerr(x, reference(a32, b, dims, sd)) < TOL
+
+
+@pytest.mark.parametrize("n", [5, 37, 100, 300])
+@pytest.mark.parametrize("sd", [1, 2])
+def test_ir_yz_padded_lengths(n, sd):
+    """y/z lengths without a whole-chunk split: the refinement kernel pads the
+    last chunk with identity rows (not the one-lane fallback)."""
+    dims = [24, n, 3] if sd == 1 else [24, 3, n]
+    print(f"ir yz padded len={n} sd={sd}", flush=True)
+    a32, b = gen32(dims, n + 7 * sd)
+    with P.Handle() as h:
+        x, steps = run_ir(a32, b, dims, sd, handle=h)
+        assert h.kernel_config(sd).startswith("ir thomas-pcr"), h.kernel_config(sd)
+    assert relerr(x, reference(a32, b, dims, sd)) < TOL
+    assert 1 <= steps <= 5
