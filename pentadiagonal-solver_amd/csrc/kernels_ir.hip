@@ -16,6 +16,13 @@
 
 namespace pentad {
 
+// right-hand side rows of one chunk, rows past the end of the system read 0
+struct VarRhsX {
+  const double *p;
+  int nv;
+  __device__ __forceinline__ double operator()(int i) const { return i < nv ? p[i] : 0.0; }
+};
+
 // Largest step count over the launch.  Every wavefront reaching one address
 // with an atomic serialises (hundreds of thousands per launch), so the
 // current value is read first and the atomic issued only when it grows --
@@ -137,6 +144,71 @@ __global__ void __launch_bounds__(MAXT)
       if (p * M + i < len) x[b0 + i * st] = xs[i];
   }
   if (t == 0) note_steps(steps_out, k);
+}
+
+// ---------------------------------------------------------------------------
+// x direction, any length: the S = blockDim / L consecutive systems of the
+// workgroup are one contiguous span of S * len elements per array; the span
+// of each coefficient array is read with coalesced loads straight into the
+// per-wavefront LDS layout [5][M][64] the refinement reads (chunk rows past
+// a system's end stay identity rows, couplings past either end are zero).
+// The right-hand side is re-read per residual from the caller's x, which is
+// written only at the end.
+// ---------------------------------------------------------------------------
+template <int M, int R>
+__global__ void __launch_bounds__(256, 2) // 3 waves per SIMD would spill for R >= 4
+    k_ir_tr_x(const float *__restrict__ ds, const float *__restrict__ dl,
+              const float *__restrict__ d, const float *__restrict__ du,
+              const float *__restrict__ dw, double *x, int64_t n_sys, int len,
+              int max_steps, double tol, int *__restrict__ steps_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int L = 1 << R;
+  constexpr int REG = 5 * M * 64;            // floats per wavefront region
+  float *lds = reinterpret_cast<float *>(smem_raw);
+  const int S = (int)blockDim.x >> R;
+  const int64_t sys0 = (int64_t)blockIdx.x * S;
+  const int ns = (int)((n_sys - sys0) < S ? (n_sys - sys0) : S);
+  const int64_t base = sys0 * len;
+  const int span = ns * len;
+  // identity everywhere first (padding rows, absent systems)
+  for (int e = threadIdx.x; e < ((int)blockDim.x >> 6) * REG; e += blockDim.x) {
+    const int k = (e % REG) / (M * 64);
+    lds[e] = (k == 2) ? 1.0f : 0.0f;
+  }
+  __syncthreads();
+  const float *src[5] = {ds, dl, d, du, dw};
+  const float inv_len = 1.0f / (float)len;
+#pragma unroll 1
+  for (int k = 0; k < 5; ++k) {
+    for (int e = threadIdx.x; e < span; e += blockDim.x) {
+      int sl = (int)((float)e * inv_len);   // e / len, then corrected
+      sl -= (sl * len > e);
+      sl += ((sl + 1) * len <= e);
+      const int r = e - sl * len;
+      const int t = sl * L + r / M, i = r % M; // owning thread, row in its chunk
+      float v = src[k][base + e];
+      if ((k == 0 && r < 2) || (k == 1 && r < 1) || (k == 3 && r + 1 >= len) ||
+          (k == 4 && r + 2 >= len))
+        v = 0.0f;
+      lds[(t >> 6) * REG + (k * M + i) * 64 + (t & 63)] = v;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sl = (int)threadIdx.x >> R, p = (int)threadIdx.x & (L - 1);
+  const bool act = sl < ns;
+  const int r0 = p * M;
+  const int nv = act ? (len - r0 <= 0 ? 0 : (len - r0 >= M ? M : len - r0)) : 0;
+  double *xrow = x + base + (act ? (int64_t)sl * len + r0 : 0);
+  WaveEx ex{lane, p, L};
+  const LdsCoef<M> A{lds + wave * REG + lane};
+  const VarRhsX b{xrow, nv};
+  double xs[M];
+  const int k = ir_solve<M, R>(A, b, xs, p, L, ex, max_steps, tol);
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+    if (i < nv) xrow[i] = xs[i];
+  if (lane == 0) note_steps(steps_out, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,6 +416,32 @@ hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int r
   IRYZ(8, 0) IRYZ(8, 1) IRYZ(8, 2) IRYZ(8, 3) IRYZ(8, 4) IRYZ(8, 5) IRYZ(8, 6) IRYZ(8, 7)
   IRYZ(10, 0) IRYZ(10, 1) IRYZ(10, 2) IRYZ(10, 3) IRYZ(10, 4) IRYZ(10, 5) IRYZ(10, 6) IRYZ(10, 7)
 #undef IRYZ
+  return hipErrorInvalidValue;
+}
+
+bool ir_tr_x_supported(int rows, int64_t len) {
+  if (rows != 8) return false;
+  int lanes = 1;
+  while ((int64_t)lanes * rows < len) lanes <<= 1;
+  return len >= 3 && lanes <= 64;
+}
+
+hipError_t launch_ir_tr_x(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                          int max_steps, double tol, int *steps, hipStream_t st) {
+  if (!ir_tr_x_supported(rows, ln.len) || ln.dir != 0) return hipErrorInvalidValue;
+  int R = 0;
+  while (((int64_t)8 << R) < ln.len) ++R;
+  const int block = 256, S = block >> R;
+  const int64_t grid = (ln.n_sys + S - 1) / S;
+  const size_t smem = (size_t)(block / 64) * 5 * 8 * 64 * sizeof(float);
+#define IRT(RR) \
+  if (R == RR) { \
+    hipLaunchKernelGGL((k_ir_tr_x<8, RR>), dim3((unsigned)grid), dim3(block), smem, st, a.ds, \
+                       a.dl, a.d, a.du, a.dw, x, ln.n_sys, (int)ln.len, max_steps, tol, steps); \
+    return hipGetLastError(); \
+  }
+  IRT(0) IRT(1) IRT(2) IRT(3) IRT(4) IRT(5) IRT(6)
+#undef IRT
   return hipErrorInvalidValue;
 }
 

@@ -283,7 +283,21 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
   if (ln.dir == 0) {
     bool al = true;
     for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
-    if (!al) return pl;
+    // lengths (or alignments) without the vector-load kernel: LDS-staged
+    auto staged = [&](Plan &q) {
+      if (!ir_tr_x_supported(8, ln.len)) return false;
+      q.kind = Plan::IrTrX;
+      q.rows = 8;
+      int lanes = 1;
+      while ((int64_t)lanes * 8 < ln.len) lanes <<= 1;
+      q.lanes = lanes;
+      return true;
+    };
+    if (want == "transpose" && staged(pl)) return pl;
+    if (!al) {
+      staged(pl);
+      return pl;
+    }
     if (frows > 0 && ir_x_supported(frows, ln.len)) {
       pl.kind = Plan::IrX;
       pl.rows = frows;
@@ -299,6 +313,7 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
         return pl;
       }
     }
+    staged(pl);
     return pl;
   }
   if (frows > 0 && ir_yz_supported(frows, fwidth, ln.len)) {
@@ -353,6 +368,9 @@ std::string plan_config(const Plan &pl) {
     return buf;
   case Plan::IrLane:
     return "ir naive";
+  case Plan::IrTrX:
+    std::snprintf(buf, sizeof buf, "ir transpose rows=%d lanes=%d", pl.rows, pl.lanes);
+    return buf;
   case Plan::SfX:
     std::snprintf(buf, sizeof buf, "shared-fact rows=%d lanes=%d", pl.rows, pl.lanes);
     return buf;
@@ -415,6 +433,8 @@ hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
   case Plan::IrYZ:
     return launch_ir_yz(a, x, ln, pl.rows, pl.sys_per_wg, max_steps, tol, steps,
                         st);
+  case Plan::IrTrX:
+    return launch_ir_tr_x(a, x, ln, pl.rows, max_steps, tol, steps, st);
   case Plan::IrLane:
     if (buffer == nullptr) return hipErrorInvalidValue;
     return launch_ir_lane(a, x, ln, buffer, max_steps, tol, steps, st);

@@ -151,6 +151,11 @@ hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
 hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int rows,
                         int sys_per_wg, int max_steps, double tol, int *steps,
                         hipStream_t st);
+// x direction, any length <= 8 * 64: coefficient spans staged through LDS
+// (rows 8, lanes the next power of two, identity rows past the end)
+bool ir_tr_x_supported(int rows, int64_t len);
+hipError_t launch_ir_tr_x(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                          int max_steps, double tol, int *steps, hipStream_t st);
 size_t ir_lane_scratch_bytes(const Lines &ln);
 hipError_t launch_ir_lane(const Diags<float> &a, double *x, const Lines &ln,
                           void *scratch, int max_steps, double tol, int *steps,

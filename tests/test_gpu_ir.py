@@ -138,3 +138,27 @@ def test_ir_yz_padded_lengths(n, sd):
         assert h.kernel_config(sd).startswith("ir thomas-pcr"), h.kernel_config(sd)
     assert relerr(x, reference(a32, b, dims, sd)) < TOL
     assert 1 <= steps <= 5
+
+
+@pytest.mark.parametrize("n", [3, 5, 37, 100, 300, 500])
+def test_ir_x_any_length(n):
+    """x lengths without a whole-chunk split: coefficient spans staged through
+    LDS into the refinement kernel (not the one-lane fallback)."""
+    dims = [n, 13, 5]
+    print(f"ir x any length len={n}", flush=True)
+    a32, b = gen32(dims, 3 * n + 1)
+    with P.Handle() as h:
+        x, steps = run_ir(a32, b, dims, 0, handle=h)
+        assert h.kernel_config(0).startswith("ir transpose"), h.kernel_config(0)
+    assert relerr(x, reference(a32, b, dims, 0)) < TOL
+    assert 1 <= steps <= 5
+
+
+def test_ir_x_forced_transpose_on_exact_length():
+    dims = [512, 9, 5]
+    a32, b = gen32(dims, 77)
+    with P.Handle() as h:
+        h.set_algo(0, "transpose")
+        x, _ = run_ir(a32, b, dims, 0, handle=h)
+        assert h.kernel_config(0) == "ir transpose rows=8 lanes=64"
+    assert relerr(x, reference(a32, b, dims, 0)) < TOL
