@@ -101,3 +101,14 @@ def test_python_bindings_match_header_prototypes():
     for name, (_, argtypes) in P._SIGNATURES.items():
         assert name in counts, name
         assert len(argtypes) == counts[name], (name, len(argtypes), counts[name])
+
+
+def test_cpp_caller_links_against_the_library():
+    """The example caller built from the reference-style C++ interface exists
+    and resolves libpentadsolver.so (no GPU needed to check the link)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(P.LIB_PATH), "adi_dropin")
+    assert os.path.exists(exe)
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    line = [l for l in out.splitlines() if "libpentadsolver.so" in l]
+    assert line and "not found" not in line[0]
