@@ -98,6 +98,15 @@ template <int M, int MAXR> struct IrFactors {
   M2<float> A0, C0;                 // MAXR == 0
 };
 
+// The FP32 factorisation takes the hardware reciprocal as it is (within an
+// ulp or two): the refinement measures and removes the error of the FP32
+// solve however it arose, so a Newton step here only costs latency.
+__device__ __forceinline__ float rcp32(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ M2<float> inv32(const M2<float> &b) {
+  const float r = rcp32(b.a00 * b.a11 - b.a01 * b.a10);
+  return {b.a11 * r, -b.a01 * r, -b.a10 * r, b.a00 * r};
+}
+
 // eliminated lower coupling of interior row i (ddl) and dw2 = w / pivot
 template <int M, int MAXR, typename Cf>
 __device__ __forceinline__ float ir_ddl(const IrFactors<M, MAXR> &F, const Cf &A, int i) {
@@ -114,14 +123,14 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
   constexpr int m = M - 2;
   const bool first = (p == 0), last = (p == P - 1);
   float dw2[m]; // kept only during the factorisation
-  F.iv[0] = recip(A.d(0));
+  F.iv[0] = rcp32(A.d(0));
   F.du2[0] = A.u(0) * F.iv[0];
   dw2[0] = A.w(0) * F.iv[0];
   F.za[0] = first ? 0.0f : -A.s(0) * F.iv[0];
   F.zb[0] = first ? 0.0f : -A.l(0) * F.iv[0];
   {
     const float l1 = A.l(1);
-    F.iv[1] = recip(A.d(1) - l1 * F.du2[0]);
+    F.iv[1] = rcp32(A.d(1) - l1 * F.du2[0]);
     F.du2[1] = (A.u(1) - l1 * dw2[0]) * F.iv[1];
     dw2[1] = A.w(1) * F.iv[1];
     F.za[1] = (-l1 * F.za[0]) * F.iv[1];
@@ -131,7 +140,7 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
   for (int i = 2; i < m; ++i) {
     const float si = A.s(i);
     const float dl_ = A.l(i) - si * F.du2[i - 2];
-    F.iv[i] = recip(A.d(i) - si * dw2[i - 2] - dl_ * F.du2[i - 1]);
+    F.iv[i] = rcp32(A.d(i) - si * dw2[i - 2] - dl_ * F.du2[i - 1]);
     F.du2[i] = (A.u(i) - dl_ * dw2[i - 1]) * F.iv[i];
     dw2[i] = A.w(i) * F.iv[i];
     F.za[i] = (-si * F.za[i - 2] - dl_ * F.za[i - 1]) * F.iv[i];
@@ -202,7 +211,7 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
   B.a11 = sb * XL1[3] + db + ub * Y0[1] + wb * Y1[1];
   C.a10 = ub * Y0[2] + wb * Y1[2];
   C.a11 = ub * Y0[3] + wb * Y1[3];
-  F.Bi0 = inv(B);
+  F.Bi0 = inv32(B);
   Am = mul(F.Bi0, Am);
   C = mul(F.Bi0, C);
   if constexpr (MAXR == 0) {
@@ -224,7 +233,7 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
       const M2<float> ACp = mul(Am, Cp), CAn = mul(C, An);
       const M2<float> Bn{1.0f - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
                          -ACp.a10 - CAn.a10, 1.0f - ACp.a11 - CAn.a11};
-      F.Bi[j] = inv(Bn);
+      F.Bi[j] = inv32(Bn);
       const M2<float> AA = mul(Am, Ap), CC = mul(C, Cn);
       Am = mul(F.Bi[j], M2<float>{-AA.a00, -AA.a01, -AA.a10, -AA.a11});
       C = mul(F.Bi[j], M2<float>{-CC.a00, -CC.a01, -CC.a10, -CC.a11});
@@ -277,7 +286,7 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
       const M2<float> ACp = mul(Am, Cp), CAn = mul(C, An);
       const M2<float> Bn{1.0f - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
                          -ACp.a10 - CAn.a10, 1.0f - ACp.a11 - CAn.a11};
-      const M2<float> Bi = inv(Bn);
+      const M2<float> Bi = inv32(Bn);
       const float t0 = q0 - (Am.a00 * rp[8] + Am.a01 * rp[9]) - (C.a00 * rn[8] + C.a01 * rn[9]);
       const float t1 = q1 - (Am.a10 * rp[8] + Am.a11 * rp[9]) - (C.a10 * rn[8] + C.a11 * rn[9]);
       const M2<float> AA = mul(Am, Ap), CC = mul(C, Cn);
