@@ -4,7 +4,8 @@
 // partition_core.hpp (same chunks, separators and 2x2 block reduced system):
 //
 //   factor A (FP32) once: interior LU, separator spikes, and the block
-//                         PCR multipliers of every round
+//                         PCR multipliers of every round that is not
+//                         negligible (truncated PCR, partition_core.hpp)
 //   x0  = solve32(A, fp32(b))
 //   repeat:  r  = b - A x      (FP64, A widened exactly from FP32)
 //            dx = solve32(A, fp32(r))
@@ -83,20 +84,29 @@ __device__ __forceinline__ void ir_zero_out_of_range(int k, float (&t)[M], bool 
 }
 
 // Factorisation of one chunk: du2, 1/pivot, the previous-separator spikes
-// (za, zb) and the block PCR multipliers of each round (MAXR >= ceil(log2 P)).
-// MAXR = 0 stores only the normalised first block row (A0, C0) and redoes
-// the block PCR inside every apply: 12 registers per round are too many for
-// long systems (hundreds of chunks), and the rounds cost arithmetic only.
+// (za, zb) and the block PCR multipliers (Ah, Ch, Bi) of up to MAXR rounds.
+// Rounds are truncated once their couplings are negligible (pcr_negligible);
+// a system that still needs rounds after the MAXR stored ones keeps the
+// normalised block row reached there (A0, C0) and every apply redoes those
+// rounds on the fly (more = true).  12 registers per stored round: MAXR = 3
+// covers diagonally dominant systems of up to 64 chunks of 8 rows (they
+// stop after 2 or 3 rounds) without pushing the kernels off 3 waves per
+// SIMD; MAXR = 0 (long systems, hundreds of chunks) redoes every round.
 // The separator rows' couplings are read from the coefficients when used
 // (the last chunk's out-of-range ones are zero there).
 template <int M, int MAXR> struct IrFactors {
   static constexpr int m = M - 2;
   static constexpr int NR = MAXR > 0 ? MAXR : 1;
   float du2[m], iv[m], za[m], zb[m];
+  int nr;    // stored rounds in use
+  bool more; // rounds after the stored ones are needed (uniform over the system)
   M2<float> Bi0;
-  M2<float> Ah[NR], Ch[NR], Bi[NR]; // MAXR > 0
-  M2<float> A0, C0;                 // MAXR == 0
+  M2<float> Ah[NR], Ch[NR], Bi[NR];
+  M2<float> A0, C0; // block row after the stored rounds (more == true)
 };
+
+// Largest number of stored PCR rounds the kernels use (see IrFactors).
+constexpr int ir_stored_rounds(int R) { return R < 3 ? R : 3; }
 
 // The FP32 factorisation takes the hardware reciprocal as it is (within an
 // ulp or two): the refinement measures and removes the error of the FP32
@@ -214,31 +224,37 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
   F.Bi0 = inv32(B);
   Am = mul(F.Bi0, Am);
   C = mul(F.Bi0, C);
-  if constexpr (MAXR == 0) {
-    F.A0 = Am;
-    F.C0 = C;
-    return;
-  }
+  F.nr = 0;
+  bool done = (P <= 1);
 #pragma unroll
   for (int j = 0; j < MAXR; ++j) {
     const int h = 1 << j;
-    if (h < P) {
-      float send[8] = {Am.a00, Am.a01, Am.a10, Am.a11, C.a00, C.a01, C.a10, C.a11};
-      float rp[8], rn[8];
-      ex.template swapc<8>(send, rp, rn, h);
-      const M2<float> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
-      const M2<float> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
-      F.Ah[j] = Am;
-      F.Ch[j] = C;
-      const M2<float> ACp = mul(Am, Cp), CAn = mul(C, An);
-      const M2<float> Bn{1.0f - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
-                         -ACp.a10 - CAn.a10, 1.0f - ACp.a11 - CAn.a11};
-      F.Bi[j] = inv32(Bn);
-      const M2<float> AA = mul(Am, Ap), CC = mul(C, Cn);
-      Am = mul(F.Bi[j], M2<float>{-AA.a00, -AA.a01, -AA.a10, -AA.a11});
-      C = mul(F.Bi[j], M2<float>{-CC.a00, -CC.a01, -CC.a10, -CC.a11});
+    if (!done) {
+      if (h >= P || (j > 0 && ex.all(pcr_negligible(Am, C)))) {
+        done = true;
+      } else {
+        F.nr = j + 1;
+        float send[8] = {Am.a00, Am.a01, Am.a10, Am.a11, C.a00, C.a01, C.a10, C.a11};
+        float rp[8], rn[8];
+        ex.template swapc<8>(send, rp, rn, h);
+        const M2<float> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
+        const M2<float> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
+        F.Ah[j] = Am;
+        F.Ch[j] = C;
+        const M2<float> ACp = mul(Am, Cp), CAn = mul(C, An);
+        const M2<float> Bn{1.0f - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
+                           -ACp.a10 - CAn.a10, 1.0f - ACp.a11 - CAn.a11};
+        F.Bi[j] = inv32(Bn);
+        const M2<float> AA = mul(Am, Ap), CC = mul(C, Cn);
+        Am = mul(F.Bi[j], M2<float>{-AA.a00, -AA.a01, -AA.a10, -AA.a11});
+        C = mul(F.Bi[j], M2<float>{-CC.a00, -CC.a01, -CC.a10, -CC.a11});
+      }
     }
   }
+  if (!done) done = ((1 << MAXR) >= P) || ex.all(pcr_negligible(Am, C));
+  F.more = !done;
+  F.A0 = Am;
+  F.C0 = C;
 }
 
 // Solve A y = f with the factors; f (M rows of this chunk) is overwritten by y.
@@ -275,7 +291,20 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
   const float r1 = f[m + 1] - A.s(m + 1) * xl1 - A.u(m + 1) * y0 - A.w(m + 1) * y1;
   float q0 = F.Bi0.a00 * r0 + F.Bi0.a01 * r1;
   float q1 = F.Bi0.a10 * r0 + F.Bi0.a11 * r1;
-  if constexpr (MAXR == 0) { // block PCR on (A, C, q) together, as in part_solve
+#pragma unroll
+  for (int j = 0; j < MAXR; ++j) {
+    const int h = 1 << j;
+    if (j < F.nr) {
+      float send[2] = {q0, q1}, rp[2], rn[2];
+      ex.template swapc<2>(send, rp, rn, h);
+      const M2<float> &Ah = F.Ah[j], &Ch = F.Ch[j], &Bi = F.Bi[j];
+      const float t0 = q0 - (Ah.a00 * rp[0] + Ah.a01 * rp[1]) - (Ch.a00 * rn[0] + Ch.a01 * rn[1]);
+      const float t1 = q1 - (Ah.a10 * rp[0] + Ah.a11 * rp[1]) - (Ch.a10 * rn[0] + Ch.a11 * rn[1]);
+      q0 = Bi.a00 * t0 + Bi.a01 * t1;
+      q1 = Bi.a10 * t0 + Bi.a11 * t1;
+    }
+  }
+  if (F.more) { // rounds past the stored ones: block PCR on (A, C, q) together
     M2<float> Am = F.A0, C = F.C0;
     auto round = [&](int h) {
       float send[10] = {Am.a00, Am.a01, Am.a10, Am.a11, C.a00, C.a01, C.a10, C.a11, q0, q1};
@@ -295,20 +324,10 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
       q0 = Bi.a00 * t0 + Bi.a01 * t1;
       q1 = Bi.a10 * t0 + Bi.a11 * t1;
     };
-    if (P > 1) round(1);
-    for (int h = 2; h < P; h <<= 1) round(h);
-  }
-#pragma unroll
-  for (int j = 0; j < MAXR; ++j) {
-    const int h = 1 << j;
-    if (h < P) {
-      float send[2] = {q0, q1}, rp[2], rn[2];
-      ex.template swapc<2>(send, rp, rn, h);
-      const M2<float> &Ah = F.Ah[j], &Ch = F.Ch[j], &Bi = F.Bi[j];
-      const float t0 = q0 - (Ah.a00 * rp[0] + Ah.a01 * rp[1]) - (Ch.a00 * rn[0] + Ch.a01 * rn[1]);
-      const float t1 = q1 - (Ah.a10 * rp[0] + Ah.a11 * rp[1]) - (Ch.a10 * rn[0] + Ch.a11 * rn[1]);
-      q0 = Bi.a00 * t0 + Bi.a01 * t1;
-      q1 = Bi.a10 * t0 + Bi.a11 * t1;
+    round(1 << MAXR); // not negligible: that is why more is set
+    for (int h = 2 << MAXR; h < P; h <<= 1) {
+      if (ex.all(pcr_negligible(Am, C))) break;
+      round(h);
     }
   }
   float s0, s1;

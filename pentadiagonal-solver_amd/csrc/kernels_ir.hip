@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
   WaveEx ex{lane, p, L};
   const LdsCoef<M> A{alds};
   const MemRhs b{bin + off, 1};
-  const int k = ir_solve<M, R>(A, b, xs, p, L, ex, max_steps, tol);
+  const int k = ir_solve<M, ir_stored_rounds(R)>(A, b, xs, p, L, ex, max_steps, tol);
   if constexpr (ROT) {
     stage_rotated_store<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, R,
                                    (int)len, out, n_sys, blk, map);
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(MAXT)
   LdsEx ex{smem_raw, sl, p, S, P};
   const RegCoef<M> A{s, l, dd, u, w};
   const RegRhs<M> bv{b};
-  const int k = ir_solve<M, R>(A, bv, xs, p, P, ex, max_steps, tol);
+  const int k = ir_solve<M, ir_stored_rounds(R)>(A, bv, xs, p, P, ex, max_steps, tol);
   if (valid) {
 #pragma unroll
     for (int i = 0; i < M; ++i)
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256, 2) // 3 waves per SIMD would spill for R 
   const LdsCoef<M> A{lds + wave * REG + lane};
   const VarRhsX b{xrow, nv};
   double xs[M];
-  const int k = ir_solve<M, R>(A, b, xs, p, L, ex, max_steps, tol);
+  const int k = ir_solve<M, ir_stored_rounds(R)>(A, b, xs, p, L, ex, max_steps, tol);
 #pragma unroll
   for (int i = 0; i < M; ++i)
     if (i < nv) xrow[i] = xs[i];
