@@ -63,8 +63,15 @@ __device__ __forceinline__ void tile_flush(const T *tile, int sshift, int len, T
   if (sys >= n_sys) return;
   T *o = out + map.lo(sys) * map.out_lo + map.hi(sys) * map.out_hi;
   const int rstep = (int)blockDim.x >> sshift; // >= 1 here
-  for (int row = (int)threadIdx.x >> sshift; row < len; row += rstep)
-    o[(int64_t)row * map.out_row] = tile[row * S + s + row / M];
+  if (S * (int)sizeof(T) >= 128) {
+    // whole 128-byte lines: streaming stores, so that the solution does not
+    // displace lines still being read from L2
+    for (int row = (int)threadIdx.x >> sshift; row < len; row += rstep)
+      __builtin_nontemporal_store(tile[row * S + s + row / M], o + (int64_t)row * map.out_row);
+  } else {
+    for (int row = (int)threadIdx.x >> sshift; row < len; row += rstep)
+      o[(int64_t)row * map.out_row] = tile[row * S + s + row / M];
+  }
 }
 
 template <typename T, int M>
