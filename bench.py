@@ -463,7 +463,7 @@ def main():
                     help="refinement tolerance of the hyperdiffusion run: its 100 steps "
                          "compound the per-solve error on a field that decays 30x, so each "
                          "solve is refined to the FP64 limit (1e-13 leaves 2.5e-12 after "
-                         "100 steps, 1e-15 leaves 1.2e-14 and costs 5% more time)")
+                         "100 steps, 1e-15 leaves 1.2e-14 and costs 5 percent more time)")
     ap.add_argument("--c5-cpu-steps", type=int, default=2,
                     help="hyperdiffusion steps re-run on the CPU oracle for the error check")
     ap.add_argument("--probe", action="store_true",
