@@ -402,6 +402,26 @@ def workload_c5(P, torch, W, h, check, steps=100, cpu_steps=2, max_refine=5, tol
         res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg}
         if refine is not None:
             res[mode]["refine_steps_max"] = refine
+    # structured coefficients: the same operators generated in the kernels
+    # from one scale per line (16 B per row FP64, 8 B FP32, against 56 / 36)
+    scales = W.hyperdiffusion_scales_torch(dims, 7)
+    for mode, sc in (("stencil_fp64", scales), ("stencil_fp32", [t.float() for t in scales])):
+        x = b.clone() if mode == "stencil_fp64" else b.float()
+        with P.AdiStencil(dims, sc, W.HYPERDIFFUSION_WEIGHTS, 1.0, handle=h) as adi:
+            ms = graph_ms_per_step(torch, adi, x, x.clone(), steps)
+            x.copy_(b)
+            adi.capture(x, steps)
+            adi.replay(1)
+            torch.cuda.synchronize()
+            cfg = [adi.config(k) for k in range(3)]
+        res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg,
+                     "bytes_per_row": 16 if mode == "stencil_fp64" else 8,
+                     "max_rel_err_vs_assembled_fp64_after_steps": float(
+                         ((x.double() - out["fp64"]).abs().max() /
+                          out["fp64"].abs().max()).item())}
+        del x
+    del scales
+    res["stencil_fp64_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["stencil_fp64"]["ms_per_step"]
     # the refinement run solves the FP32-stored operator: its solver error after
     # all steps is measured against an FP64 run of that same (widened) operator
     xw = b.clone()

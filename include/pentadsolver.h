@@ -233,6 +233,24 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t handle,
                                        const void *const *coeffs,
                                        int coeff_sets, int coeff_elem_bytes,
                                        int max_refine, double tol);
+/* ADI driver over structured coefficients (see
+ * pentadsolver_D_gpsv_stencil_batch): along every line of direction d the
+ * system is id_weights[d] * I + s * weights[d*5 .. d*5+4] (ds, dl, d - id,
+ * du, dw), couplings past the ends of the line dropped, with s the line's
+ * entry of scales[d].  scales: ndim device arrays, direction d's holding one
+ * value per line laid out over the remaining axes in the standard order
+ * (fastest first), copied at create; weights, id_weights on the host.
+ *   mode 0: FP64 field and scales; mode 1: FP32 both.
+ * The kernels generate the diagonals from the scales, so a step streams only
+ * the field (16 B per row and direction in FP64).  Every length must split
+ * into 2..64 chunks of 4..16 rows.  Step, capture, replay, timing and
+ * destroy are the pentadsolver_adi_* calls.  No reference counterpart. */
+pentadStatus_t pentadsolver_adi_create_stencil(pentadsolver_handle_t handle,
+                                               pentadsolver_adi_t *adi, int mode,
+                                               int ndim, const int *dims,
+                                               const void *const *scales,
+                                               const double *weights,
+                                               const double *id_weights);
 /* nsteps ADI steps on x, launched on the handle's stream (asynchronous). */
 pentadStatus_t pentadsolver_adi_step(pentadsolver_adi_t adi, void *x,
                                      int nsteps);
@@ -254,6 +272,34 @@ int pentadsolver_adi_refine_steps(pentadsolver_adi_t adi);
 /* Kernel configuration used along direction dir. */
 const char *pentadsolver_adi_config(pentadsolver_adi_t adi, int dir);
 void pentadsolver_adi_destroy(pentadsolver_adi_t *adi);
+
+/* Structured coefficients: solve in place along t_solvedim (shape contract
+ * of pentadsolver_D_gpsv_batch) the systems
+ *     id_weight * I + scale[line] * [w0 w1 w2 w3 w4]
+ * i.e. ds = s*w0, dl = s*w1, d = id_weight + s*w2, du = s*w3, dw = s*w4,
+ * with the couplings that reach past either end of a line dropped.
+ * scale (device): one value per line, laid out over the remaining axes in
+ * the standard order (fastest first); weights (host, 5 values) and
+ * id_weight as doubles for both precisions.  Implicit hyperdiffusion
+ * I + nu dt/h^4 [1,-4,6,-4,1] is weights {1,-4,6,-4,1}, id_weight 1.
+ * The diagonals are generated inside the kernels, so a solve reads the
+ * right-hand side and writes the solution only (16 B per row in FP64 against
+ * 56 B for five diagonal arrays).  Asynchronous on the handle's stream; no
+ * scratch.  No reference counterpart: the reference assembles and reads
+ * five diagonal arrays for every solve (its apps/preproc.hpp:248 builds
+ * them). */
+pentadStatus_t pentadsolver_D_gpsv_stencil_batch(pentadsolver_handle_t handle,
+                                                 const double *scale,
+                                                 const double *weights,
+                                                 double id_weight, double *x,
+                                                 const int *t_dims, int t_ndim,
+                                                 int t_solvedim);
+pentadStatus_t pentadsolver_S_gpsv_stencil_batch(pentadsolver_handle_t handle,
+                                                 const float *scale,
+                                                 const double *weights,
+                                                 double id_weight, float *x,
+                                                 const int *t_dims, int t_ndim,
+                                                 int t_solvedim);
 
 /* Name and tuning parameters of the kernel the last solve of this handle
  * along dir launched, e.g. "thomas-pcr rows=8 lanes=32". */

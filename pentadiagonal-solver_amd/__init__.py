@@ -52,6 +52,9 @@ _SIGNATURES = {
     "pentadsolver_clear_tuning": (None, [_P]),
     "pentadsolver_probe_stream": (_I, [_P, ctypes.POINTER(_P), _P, _SZ, _I, _I]),
     "pentadsolver_adi_create": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), _I, _I, _I, ctypes.c_double]),
+    "pentadsolver_adi_create_stencil": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "pentadsolver_D_gpsv_stencil_batch": (_I, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_double, _P, _IP, _I, _I]),
+    "pentadsolver_S_gpsv_stencil_batch": (_I, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_double, _P, _IP, _I, _I]),
     "pentadsolver_adi_step": (_I, [_P, _P, _I]),
     "pentadsolver_adi_graph_capture": (_I, [_P, _P, _I]),
     "pentadsolver_adi_graph_launch": (_I, [_P, _I]),
@@ -370,6 +373,59 @@ class Adi:
             self.close()
         except Exception:
             pass
+
+
+class AdiStencil(Adi):
+    """ADI driver over structured coefficients (pentadsolver_adi_create_stencil).
+
+    Along every line of direction d the system is
+    id_weights[d] * I + s * weights[d] (ds, dl, d - id, du, dw), couplings
+    past the ends of the line dropped, with s the line's entry of scales[d]:
+    a device tensor with one value per line over the remaining axes in the
+    standard order.  float64 scales give the FP64 driver, float32 the FP32
+    one (field of the same dtype).  weights: ndim sequences of 5 floats (or
+    one for every direction), id_weights: ndim floats (or one).
+    """
+
+    def __init__(self, dims, scales, weights, id_weights=1.0, handle=None):
+        lib = load_library()
+        self._a = _P()
+        self.ndim = len(dims)
+        if len(scales) != self.ndim:
+            raise ValueError("one scale tensor per direction")
+        eb = scales[0].element_size()
+        if any(t.element_size() != eb for t in scales) or eb not in (4, 8):
+            raise TypeError("scales must all be float32 or all float64")
+        w = [list(weights)] * self.ndim if not hasattr(weights[0], "__len__") else \
+            [list(v) for v in weights]
+        ids = [float(id_weights)] * self.ndim if not hasattr(id_weights, "__len__") else \
+            [float(v) for v in id_weights]
+        if len(w) != self.ndim or any(len(v) != 5 for v in w) or len(ids) != self.ndim:
+            raise ValueError("weights: 5 per direction; id_weights: one per direction")
+        wa = (ctypes.c_double * (5 * self.ndim))(*[float(v) for r in w for v in r])
+        ia = (ctypes.c_double * self.ndim)(*ids)
+        ptrs = (_P * self.ndim)(*[_ptr(t) for t in scales])
+        _check(lib.pentadsolver_adi_create_stencil(_hval(handle), ctypes.byref(self._a),
+                                                   0 if eb == 8 else 1, self.ndim,
+                                                   _dims_arg(dims), ptrs, wa, ia))
+
+
+def gpsv_stencil_batch(scale, weights, id_weight, x, dims: Sequence[int], solvedim: int,
+                       handle=None) -> None:
+    """Structured-coefficient solve in place along solvedim
+    (pentadsolver_{D,S}_gpsv_stencil_batch): every line's system is
+    id_weight * I + scale[line] * weights, couplings past the ends dropped.
+    scale: device tensor (x's dtype), one value per line over the remaining
+    axes in the standard order; weights: 5 floats."""
+    lib = load_library()
+    if scale.dtype != x.dtype:
+        raise TypeError("scale and x must share one dtype")
+    if len(weights) != 5:
+        raise ValueError("5 weights (ds, dl, d, du, dw)")
+    wa = (ctypes.c_double * 5)(*[float(v) for v in weights])
+    fn = getattr(lib, f"pentadsolver_{_prefix(x.dtype)}_gpsv_stencil_batch")
+    _check(fn(_hval(handle), _ptr(scale), wa, float(id_weight), _ptr(x), _dims_arg(dims),
+              len(dims), int(solvedim)))
 
 
 def kernel_that_ran(direction: int) -> str:

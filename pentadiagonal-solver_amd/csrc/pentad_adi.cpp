@@ -23,7 +23,10 @@
 // Modes: 0 = FP64, 1 = FP32, 2 = FP32 coefficients with an FP64 field and
 // on-chip iterative refinement (pentadsolver_DS_gpsv_batch_ir semantics),
 // 3 / 4 = shared factorisation in FP64 / FP32: each direction's system 0 is
-// factored once into a table and the steps stream only the field.
+// factored once into a table and the steps stream only the field;
+// 5 / 6 = structured coefficients in FP64 / FP32 (pentadsolver_adi_create_
+// stencil): every line's diagonals are id * I + s * w generated inside the
+// kernels from one scale per line, so the steps stream only the field.
 #include "pentad_api_internal.hpp"
 
 #include <algorithm>
@@ -77,6 +80,9 @@ struct Adi {
   void *coef[3][5] = {};
   void *coef_mem = nullptr;
   void *tab[3] = {}; // shared-fact factor tables (modes 3, 4), in coef_mem
+  void *scale[3] = {}; // structured coefficients (modes 5, 6), in coef_mem
+  double sw[3][5] = {};
+  double sid[3] = {1.0, 1.0, 1.0};
   Order corder[3]; // coefficient layout per direction
   void *field[2] = {nullptr, nullptr};
   void *field_mem = nullptr;
@@ -157,6 +163,34 @@ hipError_t solve_dir(Adi *A, int d, const void *in, void *out, bool first_step,
   const Plan &pl = A->plan[d];
   const int64_t len = A->dims[d], nsys = A->n / len;
   const XMap map = make_map(A, d, first_step, last_step);
+  if (A->mode == 5 || A->mode == 6) {
+    // scale of system (lo, hi): the caller lays out direction d's scales over
+    // the remaining axes in the standard order (fastest first)
+    const int ndim = A->ndim, X = (d + 1) % ndim, Y = (ndim == 3) ? (d + 2) % 3 : -1;
+    int64_t sa = 1, sb = 0;
+    if (Y >= 0) {
+      if (X < Y) sb = A->dims[X];
+      else { sa = A->dims[Y]; sb = 1; }
+    }
+    if (A->mode == 5) {
+      Stencil<double> stc;
+      stc.scale = (const double *)A->scale[d];
+      stc.sa = sa;
+      stc.sb = sb;
+      for (int k = 0; k < 5; ++k) stc.w[k] = A->sw[d][k];
+      stc.id = A->sid[d];
+      return launch_stencil_x_map<double>(stc, (const double *)in, (double *)out, true, nsys,
+                                          len, map, pl.rows, pl.lanes, st);
+    }
+    Stencil<float> stc;
+    stc.scale = (const float *)A->scale[d];
+    stc.sa = sa;
+    stc.sb = sb;
+    for (int k = 0; k < 5; ++k) stc.w[k] = (float)A->sw[d][k];
+    stc.id = (float)A->sid[d];
+    return launch_stencil_x_map<float>(stc, (const float *)in, (float *)out, true, nsys, len,
+                                       map, pl.rows, pl.lanes, st);
+  }
   if (A->mode == 3)
     return launch_sf_x_map<double>((const double *)A->tab[d], (const double *)in,
                                    (double *)out, true, nsys, len, map, pl.rows, st);
@@ -415,6 +449,77 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
   return PENTAD_STATUS_SUCCESS;
 }
 
+pentadStatus_t pentadsolver_adi_create_stencil(pentadsolver_handle_t h,
+                                               pentadsolver_adi_t *out, int mode, int ndim,
+                                               const int *dims, const void *const *scales,
+                                               const double *weights,
+                                               const double *id_weights) {
+  if (!out || !dims || !scales || !weights || !id_weights || (mode != 0 && mode != 1) ||
+      ndim < 2 || ndim > 3)
+    return PENTAD_STATUS_INVALID_VALUE;
+  for (int i = 0; i < ndim; ++i)
+    if (dims[i] < 3) return PENTAD_STATUS_INVALID_VALUE;
+  for (int d = 0; d < ndim; ++d)
+    if (!scales[d]) return PENTAD_STATUS_INVALID_VALUE;
+  Context *c = ctx_of(h);
+  Adi *A = new Adi();
+  A->c = c;
+  A->mode = mode == 0 ? 5 : 6;
+  A->ndim = ndim;
+  A->n = 1;
+  for (int i = 0; i < ndim; ++i) {
+    A->dims[i] = dims[i];
+    A->n *= dims[i];
+  }
+  A->cbytes = A->fbytes = (mode == 0) ? 8 : 4;
+  A->rotating = true;
+  hipStream_t st = c->stream;
+  auto fail = [&](pentadStatus_t s) {
+    free_adi(A);
+    return s;
+  };
+  // every direction on the x-like partitioned kernel (no in-place fallback)
+  for (int d = 0; d < ndim; ++d) {
+    Lines ln;
+    ln.dir = 0;
+    ln.len = A->dims[d];
+    ln.n_sys = A->n / ln.len;
+    if (!stencil_plan_x(A->fbytes, ln.len, A->plan[d])) {
+      std::fprintf(stderr, "[penta] adi stencil: no partitioned kernel for length %lld\n",
+                   (long long)ln.len);
+      return fail(PENTAD_STATUS_INVALID_VALUE);
+    }
+    for (int k = 0; k < 5; ++k) A->sw[d][k] = weights[d * 5 + k];
+    A->sid[d] = id_weights[d];
+    A->corder[d] = in_order(A, d, false);
+  }
+  // the scales: one per line of each direction, copied into the driver
+  size_t off[3] = {0, 0, 0}, total = 0;
+  for (int d = 0; d < ndim; ++d) {
+    off[d] = total;
+    total += ((size_t)(A->n / A->dims[d]) * A->fbytes + 255) & ~(size_t)255;
+  }
+  const size_t fsz = (size_t)A->n * A->fbytes;
+  if (hipMalloc(&A->coef_mem, total) != hipSuccess ||
+      hipMalloc(&A->field_mem, fsz * 2) != hipSuccess ||
+      hipMalloc((void **)&A->d_steps, sizeof(int)) != hipSuccess)
+    return fail(PENTAD_STATUS_ALLOC_FAILED);
+  A->field[0] = A->field_mem;
+  A->field[1] = (char *)A->field_mem + fsz;
+  if (launch_fill_int(A->d_steps, 0, st) != hipSuccess)
+    return fail(PENTAD_STATUS_EXECUTION_FAILED);
+  for (int d = 0; d < ndim; ++d) {
+    A->scale[d] = (char *)A->coef_mem + off[d];
+    if (hipMemcpyAsync(A->scale[d], scales[d], (size_t)(A->n / A->dims[d]) * A->fbytes,
+                       hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return fail(PENTAD_STATUS_EXECUTION_FAILED);
+    A->cfg[d] = "stencil " + plan_config(A->plan[d]);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(PENTAD_STATUS_EXECUTION_FAILED);
+  *out = A;
+  return PENTAD_STATUS_SUCCESS;
+}
+
 static bool x_ok(const Adi *A, const void *x) {
   return x && (!A->rotating || (reinterpret_cast<uintptr_t>(x) & 15u) == 0);
 }
@@ -430,7 +535,9 @@ pentadStatus_t pentadsolver_adi_step(pentadsolver_adi_t adi, void *x, int nsteps
   if (A->rotating)
     for (int d = 0; d < A->ndim; ++d)
       note_kernel(A->c, d,
-                  A->mode == 2 ? "ir" : (A->mode >= 3 ? "shared-fact" : "thomas-pcr"),
+                  A->mode == 2 ? "ir"
+                               : (A->mode >= 5 ? "stencil"
+                                               : (A->mode >= 3 ? "shared-fact" : "thomas-pcr")),
                   A->cfg[d]);
   return PENTAD_STATUS_SUCCESS;
 }

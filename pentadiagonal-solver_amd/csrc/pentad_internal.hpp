@@ -137,6 +137,26 @@ template <typename T>
 hipError_t launch_sf_yz(const T *tab, T *x, const Lines &ln, int rows, int sys_per_wg,
                         hipStream_t st);
 
+// Structured coefficients: along every line the system is
+// id * I + s * [w0 .. w4] (ds, dl, d - id, du, dw), couplings past either end
+// dropped, with one scale s per line.  x-like kernels index the scale as
+// lo * sa + hi * sb of the system's XMap split; the strided kernels by the
+// system index of the Lines decomposition.
+template <typename T> struct Stencil {
+  const T *scale = nullptr;
+  int64_t sa = 1, sb = 0;
+  T w[5] = {};
+  T id = T(1);
+};
+template <typename T>
+hipError_t launch_stencil_x_map(const Stencil<T> &stc, const T *b, T *out, bool rot,
+                                int64_t n_sys, int64_t len, const XMap &map, int rows,
+                                int lanes, hipStream_t st);
+bool stencil_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
+template <typename T>
+hipError_t launch_stencil_yz(const Stencil<T> &stc, T *x, const Lines &ln, int rows,
+                             int sys_per_wg, hipStream_t st);
+
 // FP32 coefficients + FP64 RHS/solution with on-chip iterative refinement.
 // steps (device int, may be null) receives atomicMax of the refinement
 // steps taken.

@@ -139,6 +139,28 @@ def hyperdiffusion_torch(dims: Sequence[int], seed: int = 7,
     return sets
 
 
+def hyperdiffusion_scales_torch(dims: Sequence[int], seed: int = 7,
+                                dt_over_h4: float = HYPERDIFFUSION_DT_OVER_H4, dtype=None,
+                                device="cuda"):
+    """The per-line scales s = nu*dt/h^4 of hyperdiffusion_torch (same
+    generator draws, so the same operators), one flat tensor per direction
+    over the remaining axes in the standard order -- the input of the
+    structured-coefficient solves (weights 1, -4, 6, -4, 1, identity 1)."""
+    import torch
+    dtype = dtype or torch.float64
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    out = []
+    for ax in range(len(dims)):
+        nu = torch.empty(_line_view(dims, ax), dtype=torch.float64, device=device)
+        nu.uniform_(0.5, 2.0, generator=g)
+        out.append((nu * dt_over_h4).reshape(-1).to(dtype).contiguous())
+    return out
+
+
+HYPERDIFFUSION_WEIGHTS = (1.0, -4.0, 6.0, -4.0, 1.0)
+
+
 def gaussian_field_torch(dims: Sequence[int], seed: int = 8, dtype=None, device="cuda"):
     import torch
     g = torch.Generator(device=device)
