@@ -29,7 +29,8 @@ variant; FP64 also against the reference's own CPU solver), 8192 systems of
 log-uniform length in [32, 8192] in one launch vs one call per system, and
 the 384x256x640 hyperdiffusion run (100 steps as one hipGraph, FP32+FP64
 refinement to --c5-tol vs FP64, the first --c5-cpu-steps steps also on the
-CPU).
+CPU; and the same operators as structured coefficients, generated in the
+kernels from one scale per line, FP64 and FP32).
 --no-extra skips them.
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--size 512]
