@@ -3,6 +3,9 @@
 // k_ir_x    x direction: L lanes of a wavefront per system, M rows per lane,
 //           FP32 coefficients via 16-byte loads, FP64 RHS/solution via
 //           16-byte loads.  HBM traffic per row: 5 x 4 B + 8 B in, 8 B out.
+// k_ir_xq   the same lines with the solution written rotated (ADI layouts):
+//           Q passes into one tile whose rows are whole 128-byte lines,
+//           coefficients in registers.
 // k_ir_yz   strided directions: S systems x P chunks per workgroup, LDS
 //           exchange, XCD-aware tile order so that workgroups reading the
 //           same 128-byte lines run on the same XCD (same L2).
@@ -84,6 +87,62 @@ __global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
     if (valid) RowsIO<double, M>::store(out + off, xs);
   }
   if (lane == 0) note_steps(steps_out, k);
+}
+
+// Rotated store (ADI layouts): Q passes of 256 >> R systems each solve into
+// one LDS tile, so that every row of the tile is a 128-byte line written
+// whole by this workgroup (4 FP64 systems per pass at 64 lanes: Q = 4).
+// The coefficients stay in registers (the LDS holds the 66 KB tile, two
+// workgroups per CU, 2 waves per SIMD): a tile of 4 systems with the
+// coefficients in LDS runs 3 waves per SIMD but writes 32-byte pieces of
+// lines shared with three other workgroups, which the memory system merges
+// poorly (1.01 ms against 0.91 ms for this kernel at 512^3).
+template <int M, int R, int Q>
+__global__ void __launch_bounds__(256, 2)
+    k_ir_xq(const float *__restrict__ ds, const float *__restrict__ dl,
+            const float *__restrict__ d, const float *__restrict__ du,
+            const float *__restrict__ dw, const double *bin, double *out,
+            int64_t n_sys, int64_t len, int max_steps, double tol,
+            int *__restrict__ steps_out, XMap map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int L = 1 << R;
+  constexpr int s0shift = 8 - R; // systems per pass (256 threads)
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int sl = (int)threadIdx.x >> R;
+  const int p = (int)(threadIdx.x & (L - 1));
+  const int lane = threadIdx.x & 63;
+  WaveEx ex{lane, p, L};
+  int kmax = 0;
+#pragma unroll 1
+  for (int q = 0; q < Q; ++q) {
+    const int64_t sys = ((blk * Q + q) << s0shift) + sl;
+    const int64_t sc = sys < n_sys ? sys : n_sys - 1;
+    const int64_t slo = map.lo(sc), shi = map.hi(sc);
+    const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+    const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
+    float s[M], l[M], dd[M], u[M], w[M];
+    RowsIO<float, M>::load(ds + offc, s);
+    RowsIO<float, M>::load(dl + offc, l);
+    RowsIO<float, M>::load(d + offc, dd);
+    RowsIO<float, M>::load(du + offc, u);
+    RowsIO<float, M>::load(dw + offc, w);
+    ir_zero_out_of_range<M>(0, s, p == 0, p == L - 1);
+    ir_zero_out_of_range<M>(1, l, p == 0, p == L - 1);
+    ir_zero_out_of_range<M>(3, u, p == 0, p == L - 1);
+    ir_zero_out_of_range<M>(4, w, p == 0, p == L - 1);
+    const RegCoef<M> A{s, l, dd, u, w};
+    const MemRhs b{bin + off, 1};
+    double xs[M];
+    const int k = ir_solve<M, ir_stored_rounds(R)>(A, b, xs, p, L, ex, max_steps, tol);
+    kmax = k > kmax ? k : kmax;
+    tile_put<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, Q << s0shift,
+                        (q << s0shift) + sl);
+  }
+  __syncthreads();
+  constexpr int sshift = s0shift + (Q == 4 ? 2 : (Q == 2 ? 1 : 0));
+  tile_flush<double, M>(reinterpret_cast<const double *>(smem_raw), sshift, (int)len, out,
+                        n_sys, blk << sshift, map);
+  if (lane == 0) note_steps(steps_out, kmax);
 }
 
 template <int M, int R, int MAXT>
@@ -314,6 +373,25 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
                    int64_t n_sys, int64_t len, const XMap &map, int max_steps,
                    double tol, int *steps, hipStream_t st) {
   const int block = 256; // 512 measured slower (fewer resident workgroups)
+  // passes per workgroup so that a tile row is a whole 128-byte line (10 rows
+  // per lane would spill with the coefficients in registers: they keep the
+  // LDS-coefficient kernel below)
+  if constexpr (M <= 8) if (rot) {
+    constexpr int run = (256 >> R) * 8;
+    constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
+    const int64_t per_wg = (int64_t)(256 >> R) * Q;
+    const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+    const size_t smem = rotated_tile_bytes<double>(block * Q, M, len);
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&k_ir_xq<M, R, Q>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_ir_xq<M, R, Q>), dim3((unsigned)grid), dim3(block), smem, st, a.ds,
+                       a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol, steps, map);
+    return hipGetLastError();
+  }
   const int64_t threads = n_sys << R;
   const int64_t grid = (threads + block - 1) / block;
   const size_t smem_a = (size_t)(block / 64) * 5 * M * 64 * sizeof(float);
@@ -327,19 +405,19 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
       if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_ir_x<M, R, true>), dim3((unsigned)grid), dim3(block), smem, st,
-                       a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol,
-                       steps, map);
-  } else {
-    if (smem_a > 65536) {
-      const hipError_t e = hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&k_ir_x<M, R, false>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_a);
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_ir_x<M, R, false>), dim3((unsigned)grid), dim3(block), smem_a,
-                       st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps,
-                       tol, steps, map);
+                       a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol, steps,
+                       map);
+    return hipGetLastError();
   }
+  if (smem_a > 65536) {
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_ir_x<M, R, false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_a);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_ir_x<M, R, false>), dim3((unsigned)grid), dim3(block), smem_a, st,
+                     a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol, steps,
+                     map);
   return hipGetLastError();
 }
 
