@@ -78,6 +78,67 @@ __global__ void __launch_bounds__(512)
   }
 }
 
+// Rotated store in Q > 1 passes, software-pipelined: the rows of pass q + 1
+// are loaded before pass q is solved, so the loads of one pass overlap the
+// arithmetic of the previous one.  The 64 KB tile allows two workgroups of
+// 256 threads per CU (2 waves per SIMD), which leaves up to 256 VGPRs for the
+// two sets of rows (FP64 M <= 8, FP32 M <= 16; larger chunks keep k_part_x).
+// Measured at 512^3: FP64 1.56 -> 1.44 ms, FP32 (4 passes of 8 systems
+// instead of 2 passes of 16 in 512-thread workgroups) 0.87 -> 0.73 ms.
+template <typename T, int M, int Q>
+__global__ void __launch_bounds__(256, 2)
+    k_part_xq(const T *__restrict__ ds, const T *__restrict__ dl,
+              const T *__restrict__ d, const T *__restrict__ du,
+              const T *__restrict__ dw, const T *b, T *out, int64_t n_sys,
+              int64_t len, int lshift, XMap map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int L = 1 << lshift;
+  const int s0shift = 8 - lshift; // systems per pass (256 threads)
+  const int sl = (int)threadIdx.x >> lshift;
+  const int p = (int)threadIdx.x & (L - 1);
+  WaveEx ex{(int)(threadIdx.x & 63), p, L};
+  auto load = [&](int q, T (&s)[M], T (&l)[M], T (&dd)[M], T (&u)[M], T (&w)[M],
+                  T (&f)[M]) {
+    const int64_t sys = ((blk * Q + q) << s0shift) + sl;
+    const int64_t sc = sys < n_sys ? sys : n_sys - 1;
+    const int64_t slo = map.lo(sc), shi = map.hi(sc);
+    const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+    const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
+    RowsIO<T, M>::load(ds + offc, s);
+    RowsIO<T, M>::load(dl + offc, l);
+    RowsIO<T, M>::load(d + offc, dd);
+    RowsIO<T, M>::load(du + offc, u);
+    RowsIO<T, M>::load(dw + offc, w);
+    RowsIO<T, M>::load(b + off, f);
+  };
+  T s[M], l[M], dd[M], u[M], w[M], f[M];
+  load(0, s, l, dd, u, w, f);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    T ns[M], nl[M], nd[M], nu[M], nw[M], nf[M];
+    if (q + 1 < Q) load(q + 1, ns, nl, nd, nu, nw, nf);
+    part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+    tile_put<T, M>(reinterpret_cast<T *>(smem_raw), f, p, Q << s0shift, (q << s0shift) + sl);
+    if (q + 1 < Q) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) {
+        s[i] = ns[i];
+        l[i] = nl[i];
+        dd[i] = nd[i];
+        u[i] = nu[i];
+        w[i] = nw[i];
+        f[i] = nf[i];
+      }
+    }
+  }
+  __syncthreads();
+  constexpr int qshift = Q == 4 ? 2 : (Q == 2 ? 1 : 0);
+  const int sshift = s0shift + qshift;
+  tile_flush<T, M>(reinterpret_cast<const T *>(smem_raw), sshift, (int)len, out, n_sys,
+                   blk << sshift, map);
+}
+
 // ---------------------------------------------------------------------------
 // strided (y/z) directions
 // ---------------------------------------------------------------------------
@@ -232,11 +293,34 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
     return hipGetLastError();
   }
-  // rotated stores: FP32 in 512-thread workgroups, FP64 in 256 (3 waves per
-  // SIMD fit its state); Q passes per workgroup widen each tile row to a
-  // 128-byte run (measured at 512^3, FP64 rows=8 x 64 lanes: 1.89 / 1.76 /
-  // 1.64 / 1.73 ms for Q = 1 / 2 / 4 / 8); runs already 128 bytes wide keep
-  // Q = 1 (more workgroups)
+  // rotated stores: Q passes per workgroup widen each tile row to a 128-byte
+  // run (measured at 512^3, FP64 rows=8 x 64 lanes: 1.89 / 1.76 / 1.64 /
+  // 1.73 ms for Q = 1 / 2 / 4 / 8); several passes run software-pipelined
+  // in 256-thread workgroups (k_part_xq) where both sets of rows fit
+  constexpr bool pipelined = sizeof(T) == 8 ? M <= 8 : M <= 16;
+  if constexpr (pipelined) {
+    const int run = (256 >> lshift) * (int)sizeof(T);
+    if (run < 128) {
+      const int Q = run >= 64 ? 2 : 4;
+      const int64_t per_wg = (int64_t)(256 >> lshift) * Q;
+      const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+      const size_t smem = rotated_tile_bytes<T>(256 * Q, M, len);
+      auto go = [&](auto kernel) {
+        if (smem > 65536) {
+          const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)smem);
+          if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), smem, st, a.ds, a.dl,
+                           a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+        return hipGetLastError();
+      };
+      return Q == 2 ? go(k_part_xq<T, M, 2>) : go(k_part_xq<T, M, 4>);
+    }
+  }
+  // one pass, or chunks too large to hold two sets of rows: FP32 in
+  // 512-thread workgroups, FP64 in 256 (3 waves per SIMD fit its state)
   const int block = sizeof(T) == 4 ? 512 : 256;
   const int run = (block >> lshift) * (int)sizeof(T); // bytes per pass
   const int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
