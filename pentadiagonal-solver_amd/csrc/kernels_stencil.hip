@@ -63,22 +63,41 @@ __global__ void __launch_bounds__(512)
   const int sl = (int)threadIdx.x >> lshift;
   const int p = (int)threadIdx.x & (L - 1);
   WaveEx ex{(int)(threadIdx.x & 63), p, L};
-#pragma unroll 1
-  for (int q = 0; q < Q; ++q) {
+  // pass q's right-hand side and scale; the next pass's are loaded before
+  // pass q is solved (software pipelining across passes)
+  auto load = [&](int q, T (&f)[M], T &scale, int64_t &off, bool &valid) {
     const int64_t sys = ((blk * Q + q) << s0shift) + sl;
-    const bool valid = sys < n_sys;
+    valid = sys < n_sys;
     const int64_t sc = valid ? sys : n_sys - 1;
     const int64_t slo = map.lo(sc), shi = map.hi(sc);
-    const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
-    const T scale = st.scale[slo * st.sa + shi * st.sb];
-    T s[M], l[M], dd[M], u[M], w[M], f[M];
+    off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+    scale = st.scale[slo * st.sa + shi * st.sb];
     RowsIO<T, M>::load(b + off, f);
+  };
+  T f[M], scale;
+  int64_t off;
+  bool valid;
+  load(0, f, scale, off, valid);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    T fn[M], scale_n = T(0);
+    int64_t off_n = 0;
+    bool valid_n = false;
+    if (q + 1 < Q) load(q + 1, fn, scale_n, off_n, valid_n);
+    T s[M], l[M], dd[M], u[M], w[M];
     stencil_rows<T, M>(st, scale, (int64_t)p * M, len, s, l, dd, u, w);
     part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
     if constexpr (ROT) {
       tile_put<T, M>(reinterpret_cast<T *>(smem_raw), f, p, Q << s0shift, (q << s0shift) + sl);
     } else {
       if (valid) RowsIO<T, M>::store(out + off, f);
+    }
+    if (q + 1 < Q) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) f[i] = fn[i];
+      scale = scale_n;
+      off = off_n;
+      valid = valid_n;
     }
   }
   if constexpr (ROT) {
