@@ -85,7 +85,10 @@ __global__ void __launch_bounds__(512)
 // two sets of rows (FP64 M <= 8, FP32 M <= 16; larger chunks keep k_part_x).
 // Measured at 512^3: FP64 1.56 -> 1.44 ms, FP32 (4 passes of 8 systems
 // instead of 2 passes of 16 in 512-thread workgroups) 0.87 -> 0.73 ms.
-template <typename T, int M, int Q>
+// ROT = false: the same pipelined passes solving in place (out == b, no
+// tile; measured at 512^3: FP64 1.55 -> 1.44 ms with Q = 2, FP32 0.81 ->
+// 0.76 ms with Q = 4).
+template <typename T, int M, int Q, bool ROT>
 __global__ void __launch_bounds__(256, 2)
     k_part_xq(const T *__restrict__ ds, const T *__restrict__ dl,
               const T *__restrict__ d, const T *__restrict__ du,
@@ -98,6 +101,11 @@ __global__ void __launch_bounds__(256, 2)
   const int sl = (int)threadIdx.x >> lshift;
   const int p = (int)threadIdx.x & (L - 1);
   WaveEx ex{(int)(threadIdx.x & 63), p, L};
+  auto offset = [&](int q) {
+    const int64_t sys = ((blk * Q + q) << s0shift) + sl;
+    const int64_t sc = sys < n_sys ? sys : n_sys - 1;
+    return map.lo(sc) * map.in_lo + map.hi(sc) * map.in_hi + (int64_t)p * M;
+  };
   auto load = [&](int q, T (&s)[M], T (&l)[M], T (&dd)[M], T (&u)[M], T (&w)[M],
                   T (&f)[M]) {
     const int64_t sys = ((blk * Q + q) << s0shift) + sl;
@@ -119,7 +127,11 @@ __global__ void __launch_bounds__(256, 2)
     T ns[M], nl[M], nd[M], nu[M], nw[M], nf[M];
     if (q + 1 < Q) load(q + 1, ns, nl, nd, nu, nw, nf);
     part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
-    tile_put<T, M>(reinterpret_cast<T *>(smem_raw), f, p, Q << s0shift, (q << s0shift) + sl);
+    if constexpr (ROT) {
+      tile_put<T, M>(reinterpret_cast<T *>(smem_raw), f, p, Q << s0shift, (q << s0shift) + sl);
+    } else {
+      if (((blk * Q + q) << s0shift) + sl < n_sys) RowsIO<T, M>::store(out + offset(q), f);
+    }
     if (q + 1 < Q) {
 #pragma unroll
       for (int i = 0; i < M; ++i) {
@@ -132,11 +144,13 @@ __global__ void __launch_bounds__(256, 2)
       }
     }
   }
-  __syncthreads();
-  constexpr int qshift = Q == 4 ? 2 : (Q == 2 ? 1 : 0);
-  const int sshift = s0shift + qshift;
-  tile_flush<T, M>(reinterpret_cast<const T *>(smem_raw), sshift, (int)len, out, n_sys,
-                   blk << sshift, map);
+  if constexpr (ROT) {
+    __syncthreads();
+    constexpr int qshift = Q == 4 ? 2 : (Q == 2 ? 1 : 0);
+    const int sshift = s0shift + qshift;
+    tile_flush<T, M>(reinterpret_cast<const T *>(smem_raw), sshift, (int)len, out, n_sys,
+                     blk << sshift, map);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -286,8 +300,18 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                 int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
+  constexpr bool pipelined = sizeof(T) == 8 ? M <= 8 : M <= 16;
   if (!rot) {
     const int block = 256;
+    if constexpr (pipelined) {
+      // two (FP64) / four (FP32) software-pipelined passes per workgroup
+      constexpr int Q = sizeof(T) == 8 ? 2 : 4;
+      const int64_t per_wg = (int64_t)(block >> lshift) * Q;
+      const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+      hipLaunchKernelGGL((k_part_xq<T, M, Q, false>), dim3((unsigned)grid), dim3(block), 0,
+                         st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+      return hipGetLastError();
+    }
     const int64_t grid = (n_sys * lanes + block - 1) / block;
     hipLaunchKernelGGL((k_part_x<T, M, false, 1>), dim3((unsigned)grid), dim3(block), 0,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
@@ -297,7 +321,6 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
   // run (measured at 512^3, FP64 rows=8 x 64 lanes: 1.89 / 1.76 / 1.64 /
   // 1.73 ms for Q = 1 / 2 / 4 / 8); several passes run software-pipelined
   // in 256-thread workgroups (k_part_xq) where both sets of rows fit
-  constexpr bool pipelined = sizeof(T) == 8 ? M <= 8 : M <= 16;
   if constexpr (pipelined) {
     const int run = (256 >> lshift) * (int)sizeof(T);
     if (run < 128) {
@@ -316,7 +339,7 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                            a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
         return hipGetLastError();
       };
-      return Q == 2 ? go(k_part_xq<T, M, 2>) : go(k_part_xq<T, M, 4>);
+      return Q == 2 ? go(k_part_xq<T, M, 2, true>) : go(k_part_xq<T, M, 4, true>);
     }
   }
   // one pass, or chunks too large to hold two sets of rows: FP32 in
