@@ -97,6 +97,37 @@ void pentadsolver_S_gpsv_batch(pentadsolver_handle_t handle, const float *ds,
                                const int *t_dims, int t_ndim, int t_solvedim,
                                void *t_buffer);
 
+/* Padded arrays: the reference's CPU library addresses element (i, j, k)
+ * at i + pads[0] * (j + pads[1] * k) with pads[0] = t_dims[0] rounded up to
+ * its SIMD width (src/cpu/singlenode/pentad_cpu.cpp:611-613, used at :457,
+ * :569), and its 2-D consumer allocates nx_pad-wide rows
+ * (apps/2D_cavity_cpu.cpp:70-72, apps/preproc.hpp:248, :320).  These take that layout
+ * explicitly: t_pads[k] >= t_dims[k] are the extents in memory (t_ndim <= 3;
+ * t_pads = NULL is the dense layout).  Elements in the padding are neither
+ * read nor written.  Same solve, routing and errors as
+ * pentadsolver_{D,S}_gpsv_batch; returns the status.  The buffer-size query
+ * takes the same shape.  The x kernels with vector loads need pads[0] (and
+ * pads[0] * pads[1]) to be multiples of 16 bytes; other pitches run the
+ * one-lane kernel, which needs scratch. */
+size_t pentadsolver_D_gpsv_batch_padded_buffer_size(
+    pentadsolver_handle_t handle, const double *ds, const double *dl,
+    const double *d, const double *du, const double *dw, const double *x,
+    const int *t_dims, const int *t_pads, int t_ndim, int t_solvedim);
+size_t pentadsolver_S_gpsv_batch_padded_buffer_size(
+    pentadsolver_handle_t handle, const float *ds, const float *dl,
+    const float *d, const float *du, const float *dw, const float *x,
+    const int *t_dims, const int *t_pads, int t_ndim, int t_solvedim);
+pentadStatus_t pentadsolver_D_gpsv_batch_padded(
+    pentadsolver_handle_t handle, const double *ds, const double *dl,
+    const double *d, const double *du, const double *dw, double *x,
+    const int *t_dims, const int *t_pads, int t_ndim, int t_solvedim,
+    void *t_buffer);
+pentadStatus_t pentadsolver_S_gpsv_batch_padded(
+    pentadsolver_handle_t handle, const float *ds, const float *dl,
+    const float *d, const float *du, const float *dw, float *x,
+    const int *t_dims, const int *t_pads, int t_ndim, int t_solvedim,
+    void *t_buffer);
+
 /* Replaces pentadsolver_kernel_that_ran (src/cuda/singlenode/
  * pentad_cuda.cu:1159-1161): name of the kernel the most recent solve along
  * direction dir (0 = x, 1 = y/middle, 2 = z/outermost) actually launched,

@@ -12,6 +12,8 @@ imports it.
   from its sources by oracle/Makefile, when they were built.
 * ``read_fixture``: reader of the reference fixture text format
   (include/util/mesh.hpp:50-80).
+* ``embed_padded`` / ``extract_padded``: the padded array layout of the
+  reference's CPU library (pentad_cpu.cpp:611-613).
 """
 from __future__ import annotations
 
@@ -168,6 +170,51 @@ def ref_gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, kind: str = "avx") -> N
     l = ref_lib(kind)
     fn = l.ref_gpsv_batch_d if dt == np.float64 else l.ref_gpsv_batch_s
     fn(*_pp(ds, dl, d, du, dw, x), _dims(dims), len(dims), int(solvedim))
+
+
+# ---------------------------------------------------------------------------
+# padded layout (reference src/cpu/singlenode/pentad_cpu.cpp:611-613: element
+# (i, j, k) at i + pads[0] * (j + pads[1] * k); the last axis is not padded)
+# ---------------------------------------------------------------------------
+def ref_simd_vec(dtype) -> int:
+    """Elements per SIMD vector of the compiled reference AVX library
+    (include/pentad_simd.hpp:20-28 under -march=core-avx2: 32 bytes)."""
+    return 32 // np.dtype(dtype).itemsize
+
+
+def padded_extent(dims: Sequence[int], pads: Sequence[int]) -> int:
+    n = int(dims[-1])
+    for p in pads[:-1]:
+        n *= int(p)
+    return n
+
+
+def _padded_shape(dims, pads):
+    # numpy shape, slowest axis first; the last axis has its true extent
+    shp = [int(p) for p in pads[:-1]] + [int(dims[-1])]
+    return tuple(reversed(shp))
+
+
+def embed_padded(a: np.ndarray, dims: Sequence[int], pads: Sequence[int],
+                 fill: float) -> np.ndarray:
+    """Dense flat array (dims[0] fastest) -> flat padded array, padding = fill."""
+    out = np.full(_padded_shape(dims, pads), fill, dtype=a.dtype)
+    view = out[tuple(slice(0, int(v)) for v in reversed(dims))]
+    view[...] = a.reshape(tuple(int(v) for v in reversed(dims)))
+    return out.reshape(-1)
+
+
+def extract_padded(p: np.ndarray, dims: Sequence[int], pads: Sequence[int]) -> np.ndarray:
+    """Flat padded array -> dense flat array."""
+    v = p.reshape(_padded_shape(dims, pads))[tuple(slice(0, int(d)) for d in reversed(dims))]
+    return np.ascontiguousarray(v).reshape(-1)
+
+
+def padding_mask(dims: Sequence[int], pads: Sequence[int]) -> np.ndarray:
+    """True at the flat positions of a padded array that hold no element."""
+    m = np.ones(_padded_shape(dims, pads), dtype=bool)
+    m[tuple(slice(0, int(v)) for v in reversed(dims))] = False
+    return m.reshape(-1)
 
 
 # ---------------------------------------------------------------------------

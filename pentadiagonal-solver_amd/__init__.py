@@ -37,6 +37,10 @@ _SIGNATURES = {
     "pentadsolver_S_gpsv_batch_buffer_size_ext": (_SZ, [_P] * 7 + [_IP, _I, _I]),
     "pentadsolver_D_gpsv_batch": (None, [_P] * 7 + [_IP, _I, _I, _P]),
     "pentadsolver_S_gpsv_batch": (None, [_P] * 7 + [_IP, _I, _I, _P]),
+    "pentadsolver_D_gpsv_batch_padded_buffer_size": (_SZ, [_P] * 7 + [_IP, _IP, _I, _I]),
+    "pentadsolver_S_gpsv_batch_padded_buffer_size": (_SZ, [_P] * 7 + [_IP, _IP, _I, _I]),
+    "pentadsolver_D_gpsv_batch_padded": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _P]),
+    "pentadsolver_S_gpsv_batch_padded": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _P]),
     "pentadsolver_kernel_that_ran": (ctypes.c_char_p, [_I]),
     "pentadsolver_get_last_status": (_I, [_P]),
     "pentadsolver_set_stream": (_I, [_P, _P]),
@@ -228,6 +232,47 @@ def solve(ds, dl, d, du, dw, x, dims: Sequence[int], solvedim: int, handle=None)
         buf = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
     gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, buf, handle)
     return buf
+
+
+def padded_extent(dims: Sequence[int], pads: Sequence[int]) -> int:
+    """Elements a padded array of extents ``pads`` (>= dims) occupies: the
+    last axis needs only dims[-1] (its padding moves no element)."""
+    n = int(dims[-1])
+    for p in pads[:-1]:
+        n *= int(p)
+    return n
+
+
+def gpsv_batch_padded_buffer_size(ds, dl, d, du, dw, x, dims: Sequence[int],
+                                  pads: Sequence[int], solvedim: int, handle=None) -> int:
+    """Scratch bytes of the padded-layout solve (0 for the on-chip kernels)."""
+    if len(pads) != len(dims):
+        raise ValueError("pads must have one entry per dimension")
+    fn = getattr(load_library(), f"pentadsolver_{_prefix(x.dtype)}_gpsv_batch_padded_buffer_size")
+    return int(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+                  _dims_arg(dims), _dims_arg(pads), len(dims), int(solvedim)))
+
+
+def gpsv_batch_padded(ds, dl, d, du, dw, x, dims: Sequence[int], pads: Sequence[int],
+                      solvedim: int, buffer=None, handle=None) -> None:
+    """Solve in place along ``solvedim`` over padded arrays: element
+    (i, j, k) at i + pads[0] * (j + pads[1] * k), pads[k] >= dims[k] -- the
+    layout of the reference's CPU library (src/cpu/singlenode/
+    pentad_cpu.cpp:611-613) and its 2-D consumer
+    (apps/2D_cavity_cpu.cpp:70-72).  Every array holds at least
+    ``padded_extent(dims, pads)`` elements; the padding is neither read nor
+    written.  Raises PentadError on an error status."""
+    if len(pads) != len(dims):
+        raise ValueError("pads must have one entry per dimension")
+    need = padded_extent(dims, pads)
+    for t in (ds, dl, d, du, dw, x):
+        if t.dtype != x.dtype:
+            raise TypeError("all arrays must share one dtype")
+        if t.numel() < need:
+            raise ValueError(f"array of {t.numel()} elements; the padded layout needs {need}")
+    fn = getattr(load_library(), f"pentadsolver_{_prefix(x.dtype)}_gpsv_batch_padded")
+    _check(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+              _dims_arg(dims), _dims_arg(pads), len(dims), int(solvedim), _ptr(buffer)))
 
 
 def gpsv_batch_ir_buffer_size_ext(ds, dl, d, du, dw, x, dims: Sequence[int],

@@ -40,6 +40,31 @@ int decompose(const int *dims, int ndim, int solvedim, Lines *out) {
   return 0;
 }
 
+int decompose_padded(const int *dims, const int *pads, int ndim, int solvedim, Lines *out) {
+  if (decompose(dims, ndim, solvedim, out) != 0) return -1;
+  if (pads == nullptr) return 0;
+  if (ndim > 3) return -1; // the reference's MAXDIM (include/pentad_common.hpp:17)
+  int64_t st[3] = {1, 1, 1};
+  bool padded = false;
+  for (int i = 0; i < ndim; ++i) {
+    if (pads[i] < dims[i]) return -1;
+    if (i > 0) st[i] = st[i - 1] * pads[i - 1];
+    // padding of the last axis moves no element
+    if (i + 1 < ndim && pads[i] != dims[i]) padded = true;
+  }
+  if (!padded) return 0;
+  Lines &ln = *out;
+  ln.stride = st[solvedim];
+  // the (at most two) other axes: the lower one indexes lines within a group
+  int other[2] = {-1, -1}, no = 0;
+  for (int i = 0; i < ndim; ++i)
+    if (i != solvedim) other[no++] = i;
+  ln.n_in = no > 0 ? dims[other[0]] : 1;
+  ln.lstep = no > 0 ? st[other[0]] : 1;
+  ln.gstep = no > 1 ? st[other[1]] : 0;
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // contexts
 // ---------------------------------------------------------------------------
@@ -112,6 +137,14 @@ static bool aligned16(const void *p) {
   return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
 }
 
+// x lines for the vector-load kernels: every array 16-byte aligned, and in a
+// padded layout the line and group pitches keep each line start aligned
+static bool x_aligned(const Lines &ln, int elem_bytes, const void *const ptrs[6]) {
+  for (int i = 0; i < 6; ++i)
+    if (!aligned16(ptrs[i])) return false;
+  return ln.dense() || ((ln.lstep * elem_bytes) % 16 == 0 && (ln.gstep * elem_bytes) % 16 == 0);
+}
+
 Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
                 const void *const ptrs[6]) {
   Plan pl;
@@ -149,8 +182,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
     static const int sf32[] = {16, 8, 6, 4};
     const int *pref = is64 ? sf64 : sf32;
     if (ln.dir == 0) {
-      bool al = true;
-      for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
+      const bool al = x_aligned(ln, elem_bytes, ptrs);
       for (int k = 0; al && k < 4; ++k) {
         if (sf_x_supported(elem_bytes, pref[k], ln.len)) {
           pl.kind = Plan::SfX;
@@ -176,6 +208,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
   // LDS-staged x solve for any length (the "transpose" capability): asked
   // for by name, or the automatic choice when no partitioned shape fits
   auto plan_tr = [&](Plan &q) {
+    if (!ln.dense()) return false; // stages spans of adjacent systems
     for (int r : {8, 16}) {
       int lanes = 1;
       while ((int64_t)lanes * r < ln.len) lanes <<= 1;
@@ -191,8 +224,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
   if (ln.dir == 0 && want == "transpose" && plan_tr(pl)) return pl;
 
   if (ln.dir == 0) {
-    bool al = true;
-    for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
+    const bool al = x_aligned(ln, elem_bytes, ptrs);
     if (al && (want == "auto" || want == "thomas-pcr")) {
       if (have_tuned && tuned.kind == Plan::PartX &&
           part_x_supported(elem_bytes, tuned.rows, tuned.lanes) &&
@@ -202,7 +234,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
         pl.lanes = tuned.lanes;
         return pl;
       }
-      if (have_tuned && tuned.kind == Plan::TrX &&
+      if (have_tuned && tuned.kind == Plan::TrX && ln.dense() &&
           tr_x_supported(tuned.rows, tuned.lanes, ln.len)) {
         pl = tuned;
         return pl;
@@ -281,11 +313,10 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
     }
   }
   if (ln.dir == 0) {
-    bool al = true;
-    for (int i = 0; i < 6; ++i) al = al && aligned16(ptrs[i]);
+    const bool al = x_aligned(ln, 4, ptrs); // (FP32 pitches aligned => FP64 ones too)
     // lengths (or alignments) without the vector-load kernel: LDS-staged
     auto staged = [&](Plan &q) {
-      if (!ir_tr_x_supported(8, ln.len)) return false;
+      if (!ln.dense() || !ir_tr_x_supported(8, ln.len)) return false;
       q.kind = Plan::IrTrX;
       q.rows = 8;
       int lanes = 1;
@@ -445,14 +476,16 @@ hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
 // ---------------------------------------------------------------------------
 // the batched solve
 // ---------------------------------------------------------------------------
+// pads == nullptr: the dense layout; else the padded layout of
+// decompose_padded (extents pads[k] >= dims[k] in memory)
 template <typename T>
-pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
-                          const T *du, const T *dw, T *x, const int *dims,
-                          int ndim, int solvedim, void *buffer) {
+pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T *d,
+                                  const T *du, const T *dw, T *x, const int *dims,
+                                  const int *pads, int ndim, int solvedim, void *buffer) {
   Lines ln;
-  if (decompose(dims, ndim, solvedim, &ln) != 0 || ln.len < 3) {
-    std::fprintf(stderr, "[penta] invalid shape (ndim %d, solvedim %d)\n",
-                 ndim, solvedim);
+  if (decompose_padded(dims, pads, ndim, solvedim, &ln) != 0 || ln.len < 3) {
+    std::fprintf(stderr, "[penta] invalid shape (ndim %d, solvedim %d%s)\n",
+                 ndim, solvedim, pads ? ", padded" : "");
     return PENTAD_STATUS_INVALID_VALUE;
   }
   if (!ds || !dl || !d || !du || !dw || !x) {
@@ -554,6 +587,13 @@ pentadStatus_t gpsv_batch_ir(Context *c, const float *ds, const float *dl,
   return PENTAD_STATUS_SUCCESS;
 }
 
+template <typename T>
+pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
+                          const T *du, const T *dw, T *x, const int *dims,
+                          int ndim, int solvedim, void *buffer) {
+  return gpsv_batch_pitched<T>(c, ds, dl, d, du, dw, x, dims, nullptr, ndim, solvedim, buffer);
+}
+
 template pentadStatus_t gpsv_batch<float>(Context *, const float *, const float *,
                                           const float *, const float *, const float *,
                                           float *, const int *, int, int, void *);
@@ -565,9 +605,9 @@ template pentadStatus_t gpsv_batch<double>(Context *, const double *, const doub
 template <typename T>
 size_t buffer_size(Context *c, const T *ds, const T *dl, const T *d,
                    const T *du, const T *dw, const T *x, const int *dims,
-                   int ndim, int solvedim) {
+                   int ndim, int solvedim, const int *pads = nullptr) {
   Lines ln;
-  if (decompose(dims, ndim, solvedim, &ln) != 0) return 0;
+  if (decompose_padded(dims, pads, ndim, solvedim, &ln) != 0) return 0;
   const void *ptrs[6] = {ds, dl, d, du, dw, x};
   const Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs);
   return plan_scratch_bytes(pl, ln, (int)sizeof(T));
@@ -632,6 +672,47 @@ void pentadsolver_S_gpsv_batch(pentadsolver_handle_t h, const float *ds,
   Context *c = ctx_of(h);
   set_status(c, gpsv_batch<float>(c, ds, dl, d, du, dw, x, t_dims, t_ndim,
                                   t_solvedim, t_buffer));
+}
+
+size_t pentadsolver_D_gpsv_batch_padded_buffer_size(
+    pentadsolver_handle_t h, const double *ds, const double *dl, const double *d,
+    const double *du, const double *dw, const double *x, const int *t_dims,
+    const int *t_pads, int t_ndim, int t_solvedim) {
+  return buffer_size<double>(ctx_of(h), ds, dl, d, du, dw, x, t_dims, t_ndim, t_solvedim,
+                             t_pads);
+}
+
+size_t pentadsolver_S_gpsv_batch_padded_buffer_size(
+    pentadsolver_handle_t h, const float *ds, const float *dl, const float *d,
+    const float *du, const float *dw, const float *x, const int *t_dims,
+    const int *t_pads, int t_ndim, int t_solvedim) {
+  return buffer_size<float>(ctx_of(h), ds, dl, d, du, dw, x, t_dims, t_ndim, t_solvedim,
+                            t_pads);
+}
+
+pentadStatus_t pentadsolver_D_gpsv_batch_padded(pentadsolver_handle_t h, const double *ds,
+                                                const double *dl, const double *d,
+                                                const double *du, const double *dw,
+                                                double *x, const int *t_dims,
+                                                const int *t_pads, int t_ndim,
+                                                int t_solvedim, void *t_buffer) {
+  Context *c = ctx_of(h);
+  const pentadStatus_t s = gpsv_batch_pitched<double>(c, ds, dl, d, du, dw, x, t_dims, t_pads,
+                                                      t_ndim, t_solvedim, t_buffer);
+  set_status(c, s);
+  return s;
+}
+
+pentadStatus_t pentadsolver_S_gpsv_batch_padded(pentadsolver_handle_t h, const float *ds,
+                                                const float *dl, const float *d,
+                                                const float *du, const float *dw, float *x,
+                                                const int *t_dims, const int *t_pads,
+                                                int t_ndim, int t_solvedim, void *t_buffer) {
+  Context *c = ctx_of(h);
+  const pentadStatus_t s = gpsv_batch_pitched<float>(c, ds, dl, d, du, dw, x, t_dims, t_pads,
+                                                     t_ndim, t_solvedim, t_buffer);
+  set_status(c, s);
+  return s;
 }
 
 const char *pentadsolver_kernel_that_ran(int dir) { return ran_global(dir); }

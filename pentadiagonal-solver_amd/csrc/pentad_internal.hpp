@@ -27,18 +27,32 @@ struct Lines {
   int64_t stride = 1; // element distance between consecutive rows of a line
   int64_t n_in = 1;   // lines that are adjacent in memory (stride-1 run)
   int dir = 0;        // 0 = x (contiguous lines), 1 = middle, 2 = outermost
+  // Padded (pitched) layouts: lstep != 0 puts line `sys` at
+  // (sys / n_in) * gstep + (sys % n_in) * lstep.  lstep == 0 is the dense
+  // layout of decompose() below.
+  int64_t lstep = 0, gstep = 0;
 
   // first element of line `sys`
   __host__ __device__ __forceinline__ int64_t base(int64_t sys) const {
+    if (lstep != 0) {
+      const int64_t o = sys / n_in;
+      return o * gstep + (sys - o * n_in) * lstep;
+    }
     if (dir == 0) return sys * len;
     if (n_in == n_sys) return sys; // outermost: lines are adjacent
     const int64_t o = sys / n_in;
     return o * n_in * len + (sys - o * n_in);
   }
+  bool dense() const { return lstep == 0; }
 };
 
 // Returns 0 on success, a negative code when the shape is invalid.
 int decompose(const int *dims, int ndim, int solvedim, Lines *out);
+// The same over a padded array: element (i, j, k) of a field with extents
+// dims[] is stored at i + pads[0] * (j + pads[1] * k), pads[k] >= dims[k]
+// (the layout of src/cpu/singlenode/pentad_cpu.cpp:611-613, :457, :569).
+// pads == nullptr, or pads that change no address, give the dense layout.
+int decompose_padded(const int *dims, const int *pads, int ndim, int solvedim, Lines *out);
 
 // ---------------------------------------------------------------------------
 // Kernel argument bundle: the five diagonals (read only) and the in/out RHS.
@@ -77,12 +91,19 @@ struct XMap {
   __host__ __device__ __forceinline__ int64_t lo(int64_t s) const { return s % S1; }
   __host__ __device__ __forceinline__ int64_t hi(int64_t s) const { return s / S1; }
 };
-// lines [sys][row] contiguous, coefficients alike, in place
+// lines [sys][row] contiguous, coefficients alike, in place (padded x
+// lines: n_in lines lstep apart per group, groups gstep apart)
 inline XMap xmap_inplace(const Lines &ln) {
   XMap m;
+  m.out_row = 1;
+  if (!ln.dense()) {
+    m.S1 = ln.n_in > 0 ? ln.n_in : 1;
+    m.in_lo = m.c_lo = m.out_lo = ln.lstep;
+    m.in_hi = m.c_hi = m.out_hi = ln.gstep;
+    return m;
+  }
   m.S1 = ln.n_sys > 0 ? ln.n_sys : 1;
   m.in_lo = m.c_lo = m.out_lo = ln.len;
-  m.out_row = 1;
   return m;
 }
 // The same solves with the mapped addressing; rot = false writes the
