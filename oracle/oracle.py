@@ -14,6 +14,9 @@ imports it.
   (include/util/mesh.hpp:50-80).
 * ``embed_padded`` / ``extract_padded``: the padded array layout of the
   reference's CPU library (pentad_cpu.cpp:611-613).
+* ``cavity_pressure_coef``: the 2-D cavity consumer's assembly of the
+  pressure-correction systems (apps/preproc.hpp:248-525); with
+  ``_ref/libref_cavity.so``, the reference's own code for it.
 """
 from __future__ import annotations
 
@@ -28,6 +31,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_AVX = os.path.join(HERE, "_ref", "libref_avx.so")
 REF_ORIG = os.path.join(HERE, "_ref", "libref_orig.so")
+REF_CAVITY = os.path.join(HERE, "_ref", "libref_cavity.so")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -61,6 +65,9 @@ def lib() -> ctypes.CDLL:
         l.po_varlen_d.restype = _I
         l.po_varlen_ld.argtypes = [_P] * 7 + [_P, ctypes.c_int64]
         l.po_varlen_ld.restype = _I
+        l.po_cavity_pressure_coef_d.argtypes = ([_I, _I, _I] + [ctypes.c_double] * 3
+                                                + [_P] * 14)
+        l.po_cavity_pressure_coef_d.restype = _I
         _lib = l
     return _lib
 
@@ -170,6 +177,61 @@ def ref_gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, kind: str = "avx") -> N
     l = ref_lib(kind)
     fn = l.ref_gpsv_batch_d if dt == np.float64 else l.ref_gpsv_batch_s
     fn(*_pp(ds, dl, d, du, dw, x), _dims(dims), len(dims), int(solvedim))
+
+
+# ---------------------------------------------------------------------------
+# 2-D cavity consumer: pressure-correction assembly (reference
+# apps/preproc.hpp:248-525, pressure_coef_matrix)
+# ---------------------------------------------------------------------------
+CAVITY_OUT = ("x", "s_x", "l_x", "d_x", "u_x", "w_x", "s_y", "l_y", "d_y", "u_y", "w_y")
+
+
+def _cavity_call(fn, nx, ny, nx_pad, alphaU, dx, dy, u_prime, v_prime, uv, out):
+    ins = [np.ascontiguousarray(a, np.float64) for a in (u_prime, v_prime, uv)]
+    if ins[0].size != (ny + 2) * (nx + 1) or ins[1].size != (ny + 1) * (nx + 2) \
+            or ins[2].size != (ny + 2) * (nx + 2):
+        raise ValueError("u_prime (ny+2)x(nx+1), v_prime (ny+1)x(nx+2), uv (ny+2)x(nx+2)")
+    for k in CAVITY_OUT:
+        _c(out[k], np.float64)
+        if out[k].size < nx_pad * ny:
+            raise ValueError(f"{k}: nx_pad * ny elements needed")
+    return fn(int(nx), int(ny), int(nx_pad), float(alphaU), float(dx), float(dy),
+              *_pp(*ins), *_pp(*[out[k] for k in CAVITY_OUT]))
+
+
+def cavity_outputs(nx_pad, ny, fill=0.0):
+    return {k: np.full(nx_pad * ny, fill, dtype=np.float64) for k in CAVITY_OUT}
+
+
+def cavity_pressure_coef(nx, ny, nx_pad, alphaU, dx, dy, u_prime, v_prime, uv, out=None):
+    """po_cavity_pressure_coef_d: the reference's assembly, its write order and
+    index expressions included.  ``out`` (dict of CAVITY_OUT arrays of at
+    least nx_pad * ny elements) is written in place; elements the assembly
+    does not write keep their values."""
+    out = cavity_outputs(nx_pad, ny) if out is None else out
+    rc = _cavity_call(lib().po_cavity_pressure_coef_d, nx, ny, nx_pad, alphaU, dx, dy,
+                      u_prime, v_prime, uv, out)
+    if rc != 0:
+        raise ValueError(f"oracle rc={rc}")
+    return out
+
+
+def ref_cavity_available() -> bool:
+    return os.path.exists(REF_CAVITY)
+
+
+def ref_cavity_pressure_coef(nx, ny, nx_pad, alphaU, dx, dy, u_prime, v_prime, uv, out=None):
+    """The reference's own pressure_coef_matrix (oracle/_ref/libref_cavity.so)."""
+    if "cavity" not in _ref:
+        l = ctypes.CDLL(REF_CAVITY)
+        l.ref_cavity_pressure_coef.argtypes = ([_I, _I, _I] + [ctypes.c_double] * 3
+                                               + [_P] * 14)
+        l.ref_cavity_pressure_coef.restype = _I
+        _ref["cavity"] = l
+    out = cavity_outputs(nx_pad, ny) if out is None else out
+    _cavity_call(_ref["cavity"].ref_cavity_pressure_coef, nx, ny, nx_pad, alphaU, dx, dy,
+                 u_prime, v_prime, uv, out)
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -28,6 +28,9 @@
  *                       oracle" of the 64^3 CPU configuration).
  *   po_residual_d       max-norm backward residual |A x - b| per batch, the
  *                       check of apps/verify_strided_cuda.cu:143-173.
+ *   po_cavity_pressure_coef_d
+ *                       the 2-D cavity consumer's assembly of the pressure-
+ *                       correction systems, apps/preproc.hpp:248-525.
  *
  * Out-of-range couplings (ds of rows 0-1, dl of row 0, du of row N-1 and dw
  * of rows N-2..N-1) are ignored, as in the reference: the Thomas recurrence
@@ -443,4 +446,108 @@ int po_varlen_ld(const double *ds, const double *dl, const double *d,
                  (size_t)n, 1);
   }
   return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2-D cavity pressure-correction assembly: apps/preproc.hpp:248-525         */
+/* (pressure_coef_matrix).  Grid points (i, j), i = 1..ny (rows), j = 1..nx; */
+/* the inputs are row-major with the reference's extents: u_prime            */
+/* (ny+2) x (nx+1), v_prime (ny+1) x (nx+2), uv (ny+2) x (nx+2).  With       */
+/* c(a, b) = 0.5 * alphaU * (1/a + 1/b) the four couplings of a point are    */
+/*   E = c(uv[i][j], uv[i][j+1]) dy^2,  W = c(uv[i][j], uv[i][j-1]) dy^2,    */
+/*   N = c(uv[i][j], uv[i-1][j]) dx^2,  S = c(uv[i][j], uv[i+1][j]) dx^2,    */
+/* and the reference assigns them, per position, to (ds, dl, du, dw) with   */
+/* 1 where it assigns nothing; d = du + dl + ds + dw in that order:          */
+/*   interior, j = 1       ds 1, dl N, du E, dw S          (:273-295)        */
+/*   interior, j = nx      ds N, dl W, du 1, dw S          (:297-318)        */
+/*   interior, otherwise   ds N, dl W, du E, dw S          (:319-341)        */
+/*   top row (i = 1)       ds 1, dl W, du E, dw S          (:347-376)        */
+/*   bottom row (i = ny)   ds N, dl W, du E, dw = the top row's last dw,     */
+/*                         S of (1, nx-1): its loop does not reset the       */
+/*                         variables (:378-401)                              */
+/*   corners               (1,1) E, S; (1,nx) W, S; (ny,1) E, N; (ny,nx)    */
+/*                         W, N; the other two 1              (:403-520)     */
+/* right-hand side  x = -(u'[i][j] - u'[i][j-1]) dy - (v'[i-1][j] -          */
+/* v'[i][j]) dx (fused as compiled).  The same values go to the x set and   */
+/* the y set.  Writes                                                       */
+/* happen in the reference's order (interior rows, top, bottom, corners) at  */
+/* its index expressions, which address row (i-1) * nx_pad + (j-1) only     */
+/* when nx == nx_pad: the interior right column goes to i * nx_pad - 1, the */
+/* bottom row to (ny-1) * nx + j - 1 (:298, :380, :465, :496).  Needs        */
+/* nx, ny >= 3 (the stale dw is the top row's).  Returns 0, or -1.           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int nx, ny;
+  double aU, dx, dy;
+  const double *up, *vp, *uv;
+} po_cav;
+
+static double po_cav_c(const po_cav *g, int i0, int j0, int i1, int j1, double h2) {
+  const int ld = g->nx + 2;
+  return 0.5 * g->aU * (1.0 / g->uv[i0 * ld + j0] + 1.0 / g->uv[i1 * ld + j1]) * h2;
+}
+
+/* -(u'[i][j] - u'[i][j-1]) dy - (v'[i-1][j] - v'[i][j]) dx as the reference
+ * compiles it with its own flags (-O3 -march=core-avx2: the first product and
+ * the subtraction become one fused multiply-add). */
+static double po_cav_rhs(const po_cav *g, int i, int j) {
+  const int lu = g->nx + 1, lv = g->nx + 2;
+  const double p = -(g->up[i * lu + j] - g->up[i * lu + j - 1]);
+  const double q = g->vp[(i - 1) * lv + j] - g->vp[i * lv + j];
+  return fma(p, g->dy, -(q * g->dx));
+}
+
+static void po_cav_put(double *const out[11], int64_t idx, double xx, double s, double l,
+                       double u, double w) {
+  const double d = u + l + s + w;
+  const double v[5] = {s, l, d, u, w};
+  out[0][idx] = xx;
+  for (int k = 0; k < 5; ++k) {
+    out[1 + k][idx] = v[k];
+    if (out[6 + k]) out[6 + k][idx] = v[k];
+  }
+}
+
+int po_cavity_pressure_coef_d(int nx, int ny, int nx_pad, double alphaU, double dx,
+                              double dy, const double *u_prime, const double *v_prime,
+                              const double *uv, double *x, double *s_x, double *l_x,
+                              double *d_x, double *u_x, double *w_x, double *s_y,
+                              double *l_y, double *d_y, double *u_y, double *w_y) {
+  if (nx < 3 || ny < 3 || nx_pad < nx) return -1;
+  const po_cav g = {nx, ny, alphaU, dx, dy, u_prime, v_prime, uv};
+  double *const out[11] = {x, s_x, l_x, d_x, u_x, w_x, s_y, l_y, d_y, u_y, w_y};
+  const double dx2 = dx * dx, dy2 = dy * dy;
+#define E_(i, j) po_cav_c(&g, i, j, i, (j) + 1, dy2)
+#define W_(i, j) po_cav_c(&g, i, j, i, (j) - 1, dy2)
+#define N_(i, j) po_cav_c(&g, i, j, (i) - 1, j, dx2)
+#define S_(i, j) po_cav_c(&g, i, j, (i) + 1, j, dx2)
+  for (int i = 2; i < ny; ++i) {
+    for (int j = 1; j <= nx; ++j) {
+      const double xx = po_cav_rhs(&g, i, j);
+      if (j == 1)
+        po_cav_put(out, (int64_t)(i - 1) * nx_pad, xx, 1.0, N_(i, j), E_(i, j), S_(i, j));
+      else if (j == nx)
+        po_cav_put(out, (int64_t)i * nx_pad - 1, xx, N_(i, j), W_(i, j), 1.0, S_(i, j));
+      else
+        po_cav_put(out, (int64_t)(i - 1) * nx_pad + (j - 1), xx, N_(i, j), W_(i, j),
+                   E_(i, j), S_(i, j));
+    }
+  }
+  for (int j = 2; j < nx; ++j)
+    po_cav_put(out, j - 1, po_cav_rhs(&g, 1, j), 1.0, W_(1, j), E_(1, j), S_(1, j));
+  const double stale_w = S_(1, nx - 1);
+  for (int j = 2; j < nx; ++j)
+    po_cav_put(out, (int64_t)(ny - 1) * nx + (j - 1), po_cav_rhs(&g, ny, j), N_(ny, j),
+               W_(ny, j), E_(ny, j), stale_w);
+  po_cav_put(out, 0, po_cav_rhs(&g, 1, 1), 1.0, 1.0, E_(1, 1), S_(1, 1));
+  po_cav_put(out, nx - 1, po_cav_rhs(&g, 1, nx), 1.0, W_(1, nx), 1.0, S_(1, nx));
+  po_cav_put(out, (int64_t)(ny - 1) * nx, po_cav_rhs(&g, ny, 1), N_(ny, 1), 1.0, E_(ny, 1),
+             1.0);
+  po_cav_put(out, (int64_t)nx * ny - 1, po_cav_rhs(&g, ny, nx), N_(ny, nx), W_(ny, nx), 1.0,
+             1.0);
+#undef E_
+#undef W_
+#undef N_
+#undef S_
+  return 0;
 }

@@ -19,6 +19,13 @@ are this library's own, fixed so that tests and benchmarks are reproducible:
   symmetric positive definite with eigenvalues >= 1 and every direction
   solve is a contraction in the 2-norm.  Default dt/h^4 = 0.25.
 * ``varlen_offsets``: lengths round(exp(U(ln lo, ln hi))) packed contiguously.
+* ``cavity``: inputs of the 2-D cavity consumer's pressure-correction step
+  (reference apps/2D_cavity_cpu.cpp:183-230): face velocities u' ~ U(-1, 1)
+  on (ny+2) x (nx+1), v' ~ U(-1, 1) on (ny+1) x (nx+2), momentum diagonal
+  uv = A_p ~ U(1, 3) on (ny+2) x (nx+2) (positive, as the momentum
+  assembly's sum of couplings makes it), all row-major; dx = 1/nx,
+  dy = 1/ny, alphaU = 0.7 (the reference's velocity under-relaxation,
+  2D_cavity_cpu.cpp:123).  Draw order: u', v', uv.
 
 Host generation uses numpy (``numpy.random.default_rng(seed)``); device
 generation uses torch's CUDA generator with the same recipe (different random
@@ -178,3 +185,26 @@ def varlen_offsets(nsys: int, seed: int, lo: int = 32, hi: int = 8192) -> np.nda
     off = np.zeros(nsys + 1, dtype=np.int64)
     np.cumsum(lens, out=off[1:])
     return off
+
+
+CAVITY_ALPHA_U = 0.7
+
+
+def cavity_np(nx: int, ny: int, seed: int = 9):
+    """(u_prime, v_prime, uv, alphaU, dx, dy) of the cavity recipe, float64."""
+    rng = np.random.default_rng(seed)
+    up = rng.uniform(-1, 1, (ny + 2) * (nx + 1))
+    vp = rng.uniform(-1, 1, (ny + 1) * (nx + 2))
+    uv = rng.uniform(1, 3, (ny + 2) * (nx + 2))
+    return up, vp, uv, CAVITY_ALPHA_U, 1.0 / nx, 1.0 / ny
+
+
+def cavity_torch(nx: int, ny: int, seed: int = 9, device="cuda"):
+    """The same recipe drawn on the device (torch's generator)."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    f64 = torch.float64
+    up = torch.rand((ny + 2) * (nx + 1), dtype=f64, device=device, generator=g) * 2 - 1
+    vp = torch.rand((ny + 1) * (nx + 2), dtype=f64, device=device, generator=g) * 2 - 1
+    uv = torch.rand((ny + 2) * (nx + 2), dtype=f64, device=device, generator=g) * 2 + 1
+    return up, vp, uv, CAVITY_ALPHA_U, 1.0 / nx, 1.0 / ny
