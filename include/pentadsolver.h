@@ -128,6 +128,38 @@ pentadStatus_t pentadsolver_S_gpsv_batch_padded(
     const int *t_dims, const int *t_pads, int t_ndim, int t_solvedim,
     void *t_buffer);
 
+/* The 2-D cavity consumer's pressure-correction step on the GPU
+ * (apps/2D_cavity_cpu.cpp:195-219).  Device inputs, row-major with the
+ * reference's extents: u_prime (ny+2) x (nx+1), v_prime (ny+1) x (nx+2),
+ * uv (= A_p) (ny+2) x (nx+2); the field x and the arrays have ny rows of
+ * nx_pad elements (nx_pad >= nx; nx, ny >= 3).
+ *
+ * _coef replaces pressure_coef_matrix (apps/preproc.hpp:248-525): the
+ * right-hand side into x and the five diagonals into the x set and, when
+ * s_y..w_y are not NULL, the y set (the reference fills both with the same
+ * values).  Point (i, j) goes to (i-1) * nx_pad + (j-1); the values are the
+ * reference's bit for bit.  (When nx < nx_pad the reference's own index
+ * expressions put the right column and the bottom row elsewhere,
+ * preproc.hpp:298, :381, :465, :495; its consumer only runs nx == nx_pad,
+ * where they agree.)
+ *
+ * _solve is that assembly followed by the x solve and the y solve of
+ * apps/2D_cavity_cpu.cpp:201-219, with the coefficients generated inside the
+ * solve kernels: the solution overwrites x (its input values are ignored),
+ * no coefficient array is written or read.  nx <= 1024, ny <= 8192 (larger
+ * grids: _coef + pentadsolver_D_gpsv_batch_padded).  No reference
+ * counterpart as one call. */
+pentadStatus_t pentadsolver_D_cavity_pressure_coef(
+    pentadsolver_handle_t handle, int nx, int ny, int nx_pad, double alphaU,
+    double dx, double dy, const double *u_prime, const double *v_prime,
+    const double *uv, double *x, double *s_x, double *l_x, double *d_x,
+    double *u_x, double *w_x, double *s_y, double *l_y, double *d_y,
+    double *u_y, double *w_y);
+pentadStatus_t pentadsolver_D_cavity_pressure_solve(
+    pentadsolver_handle_t handle, int nx, int ny, int nx_pad, double alphaU,
+    double dx, double dy, const double *u_prime, const double *v_prime,
+    const double *uv, double *x);
+
 /* Replaces pentadsolver_kernel_that_ran (src/cuda/singlenode/
  * pentad_cuda.cu:1159-1161): name of the kernel the most recent solve along
  * direction dir (0 = x, 1 = y/middle, 2 = z/outermost) actually launched,

@@ -473,7 +473,7 @@ int po_varlen_ld(const double *ds, const double *dl, const double *d,
 /* happen in the reference's order (interior rows, top, bottom, corners) at  */
 /* its index expressions, which address row (i-1) * nx_pad + (j-1) only     */
 /* when nx == nx_pad: the interior right column goes to i * nx_pad - 1, the */
-/* bottom row to (ny-1) * nx + j - 1 (:298, :380, :465, :496).  Needs        */
+/* bottom row to (ny-1) * nx + j - 1 (:298, :381, :465, :495).  Needs        */
 /* nx, ny >= 3 (the stale dw is the top row's).  Returns 0, or -1.           */
 /* ------------------------------------------------------------------------ */
 typedef struct {

@@ -41,6 +41,8 @@ _SIGNATURES = {
     "pentadsolver_S_gpsv_batch_padded_buffer_size": (_SZ, [_P] * 7 + [_IP, _IP, _I, _I]),
     "pentadsolver_D_gpsv_batch_padded": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _P]),
     "pentadsolver_S_gpsv_batch_padded": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _P]),
+    "pentadsolver_D_cavity_pressure_coef": (_I, [_P, _I, _I, _I] + [ctypes.c_double] * 3 + [_P] * 14),
+    "pentadsolver_D_cavity_pressure_solve": (_I, [_P, _I, _I, _I] + [ctypes.c_double] * 3 + [_P] * 4),
     "pentadsolver_kernel_that_ran": (ctypes.c_char_p, [_I]),
     "pentadsolver_get_last_status": (_I, [_P]),
     "pentadsolver_set_stream": (_I, [_P, _P]),
@@ -273,6 +275,47 @@ def gpsv_batch_padded(ds, dl, d, du, dw, x, dims: Sequence[int], pads: Sequence[
     fn = getattr(load_library(), f"pentadsolver_{_prefix(x.dtype)}_gpsv_batch_padded")
     _check(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
               _dims_arg(dims), _dims_arg(pads), len(dims), int(solvedim), _ptr(buffer)))
+
+
+def _cavity_inputs(nx, ny, nx_pad, u_prime, v_prime, uv, x):
+    import torch
+    for t, n, name in ((u_prime, (ny + 2) * (nx + 1), "u_prime"),
+                       (v_prime, (ny + 1) * (nx + 2), "v_prime"),
+                       (uv, (ny + 2) * (nx + 2), "uv"), (x, nx_pad * ny, "x")):
+        if t.dtype != torch.float64:
+            raise TypeError(f"{name} must be float64")
+        if t.numel() < n:
+            raise ValueError(f"{name}: {n} elements needed, got {t.numel()}")
+
+
+def cavity_pressure_coef(nx, ny, nx_pad, alpha_u, dx, dy, u_prime, v_prime, uv, x,
+                         xset, yset=None, handle=None) -> None:
+    """The 2-D cavity consumer's pressure-correction assembly on the GPU (the
+    drop-in for apps/preproc.hpp:248-525, pressure_coef_matrix).
+
+    u_prime (ny+2)x(nx+1), v_prime (ny+1)x(nx+2), uv (ny+2)x(nx+2) row-major
+    float64 device tensors; x and the arrays of ``xset`` (and ``yset``, or
+    None) = (ds, dl, d, du, dw) hold ny rows of nx_pad elements.  Point
+    (i, j) goes to (i-1)*nx_pad + (j-1)."""
+    _cavity_inputs(nx, ny, nx_pad, u_prime, v_prime, uv, x)
+    arrs = list(xset) + (list(yset) if yset is not None else [None] * 5)
+    for t in arrs:
+        if t is not None:
+            _cavity_inputs(nx, ny, nx_pad, u_prime, v_prime, uv, t)
+    _check(load_library().pentadsolver_D_cavity_pressure_coef(
+        _hval(handle), int(nx), int(ny), int(nx_pad), float(alpha_u), float(dx), float(dy),
+        _ptr(u_prime), _ptr(v_prime), _ptr(uv), _ptr(x), *[_ptr(t) for t in arrs]))
+
+
+def cavity_pressure_solve(nx, ny, nx_pad, alpha_u, dx, dy, u_prime, v_prime, uv, x,
+                          handle=None) -> None:
+    """Assembly + x solve + y solve of the cavity consumer's pressure step
+    (apps/2D_cavity_cpu.cpp:195-219) with the coefficients generated inside
+    the solve kernels; the solution overwrites x (ny rows of nx_pad)."""
+    _cavity_inputs(nx, ny, nx_pad, u_prime, v_prime, uv, x)
+    _check(load_library().pentadsolver_D_cavity_pressure_solve(
+        _hval(handle), int(nx), int(ny), int(nx_pad), float(alpha_u), float(dx), float(dy),
+        _ptr(u_prime), _ptr(v_prime), _ptr(uv), _ptr(x)))
 
 
 def gpsv_batch_ir_buffer_size_ext(ds, dl, d, du, dw, x, dims: Sequence[int],

@@ -1,0 +1,259 @@
+// kernels_cavity.hip -- the 2-D cavity consumer's pressure-correction solve.
+//
+// The reference's 2-D consumer (apps/2D_cavity_cpu.cpp:195-219) assembles,
+// every iteration, a pentadiagonal system per grid row and per grid column
+// from the momentum diagonal uv = A_p and the face velocities u', v'
+// (apps/preproc.hpp:248-525, pressure_coef_matrix), writes eleven arrays (the
+// right-hand side and two identical sets of five diagonals), then solves
+// along x and along y in place on the padded field (rows of nx_pad).
+//
+// cavity_point() is that assembly for one grid point.  Three kernels use it:
+//   k_cavity_assemble  writes the arrays (the drop-in for the assembly);
+//   k_cavity_x         the x solve with the coefficients and right-hand side
+//                      generated in registers (partitioned solve, lanes of a
+//                      row exchange through the wavefront);
+//   k_cavity_y         the y solve with the coefficients generated in
+//                      registers and the right-hand side the x solve left
+//                      (S columns x P chunks per workgroup, LDS exchange).
+// The fused pair reads uv, u' and v' (24 B per point) and the field once
+// more for y: 56 B per point for assembly + both solves, against 184 B when
+// the arrays are written and read back.
+//
+// Arithmetic follows the reference's expressions in their order, with
+// contraction off, except the right-hand side, which the reference's build
+// (-O3 -march=core-avx2) fuses into one multiply-add: the values are the
+// reference's bit for bit.
+#include "partition_core.hpp"
+#include "partition_exchange.hpp"
+#include "pentad_internal.hpp"
+
+namespace pentad {
+
+// The five entries (ds, dl, d, du, dw) and right-hand side of point (i, j),
+// 1 <= i <= ny, 1 <= j <= nx (nx, ny >= 3).  With
+// c(a, b) = 0.5 * alphaU * (1/a + 1/b) the couplings are E = c(uv_ij,
+// uv_i,j+1) dy^2, W = c(uv_ij, uv_i,j-1) dy^2, N = c(uv_ij, uv_i-1,j) dx^2,
+// S = c(uv_ij, uv_i+1,j) dx^2, assigned per position as the reference does
+// (1 where it assigns nothing, d = du + dl + ds + dw):
+//   interior j = 1: ds 1, dl N, du E, dw S        (preproc.hpp:273-295)
+//   interior j = nx: ds N, dl W, du 1, dw S       (:297-318)
+//   interior: ds N, dl W, du E, dw S              (:319-341)
+//   top row: ds 1, dl W, du E, dw S               (:347-376)
+//   bottom row: ds N, dl W, du E, dw = S of (1, nx-1), the value its loop
+//     inherits from the top row's last point     (:378-401)
+//   corners (1,1): E, S; (1,nx): W, S; (ny,1): E, N; (ny,nx): W, N  (:403-520)
+__device__ __forceinline__ double cav_c(double ia, double ib, double half_au, double h2) {
+#pragma clang fp contract(off)
+  return half_au * (ia + ib) * h2;
+}
+
+__device__ __forceinline__ void cavity_point(const Cavity &c, int i, int j, double &ds,
+                                             double &dl, double &d, double &du, double &dw,
+                                             double &rhs) {
+#pragma clang fp contract(off)
+  const int lu = c.nx + 1, lv = c.nx + 2;
+  const double *uv = c.uv + (int64_t)i * lv + j;
+  const double half_au = 0.5 * c.alpha_u;
+  const double dx2 = c.dx * c.dx, dy2 = c.dy * c.dy;
+  const double ic = 1.0 / uv[0];
+  const double E = cav_c(ic, 1.0 / uv[1], half_au, dy2);
+  const double W = cav_c(ic, 1.0 / uv[-1], half_au, dy2);
+  const double N = cav_c(ic, 1.0 / uv[-lv], half_au, dx2);
+  const double S = cav_c(ic, 1.0 / uv[lv], half_au, dx2);
+  const bool top = i == 1, bot = i == c.ny, left = j == 1, right = j == c.nx;
+  double s = N, l = W, u = E, w = S;
+  if (top) {
+    s = 1.0;
+    if (left) l = 1.0;
+    if (right) u = 1.0;
+  } else if (bot) {
+    if (left) { l = 1.0; w = 1.0; }
+    else if (right) { u = 1.0; w = 1.0; }
+    else {
+      const double *u1 = c.uv + lv + (c.nx - 1); // (1, nx - 1)
+      w = cav_c(1.0 / u1[0], 1.0 / u1[lv], half_au, dx2);
+    }
+  } else if (left) {
+    s = 1.0;
+    l = N;
+  } else if (right) {
+    u = 1.0;
+  }
+  ds = s;
+  dl = l;
+  du = u;
+  dw = w;
+  d = u + l + s + w;
+  const double p = -(c.up[(int64_t)i * lu + j] - c.up[(int64_t)i * lu + j - 1]);
+  const double q = c.vp[(int64_t)(i - 1) * lv + j] - c.vp[(int64_t)i * lv + j];
+  rhs = __builtin_fma(p, c.dy, -(q * c.dx));
+}
+
+// one thread per grid point; point (i, j) to (i-1) * nx_pad + (j-1)
+__global__ void __launch_bounds__(256) k_cavity_assemble(Cavity c, CavityOut o) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)c.nx * c.ny) return;
+  const int i = (int)(t / c.nx) + 1, j = (int)(t % c.nx) + 1;
+  double v[6];
+  cavity_point(c, i, j, v[1], v[2], v[3], v[4], v[5], v[0]);
+  const int64_t k = (int64_t)(i - 1) * c.nx_pad + (j - 1);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) o.p[a][k] = v[a];
+  if (o.p[6] != nullptr) {
+#pragma unroll
+    for (int a = 1; a < 6; ++a) o.p[5 + a][k] = v[a];
+  }
+}
+
+// rows r0 .. r0+M-1 of the line through (i, j) along x (LINE_X) or y: the
+// reference's entries, couplings past either end dropped, rows past the end
+// identity rows with right-hand side 0
+template <int M, bool LINE_X>
+__device__ __forceinline__ void cavity_rows(const Cavity &c, int fixed, int r0, double (&s)[M],
+                                            double (&l)[M], double (&d)[M], double (&u)[M],
+                                            double (&w)[M], double (&f)[M]) {
+  const int len = LINE_X ? c.nx : c.ny;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int r = r0 + k;
+    // rows past the end evaluate a real point (the line's last) and are
+    // then overwritten by identity rows: no divergent branch around the
+    // generator
+    const int rc = r < len ? r : len - 1;
+    double rhs;
+    if constexpr (LINE_X)
+      cavity_point(c, fixed, rc + 1, s[k], l[k], d[k], u[k], w[k], rhs);
+    else
+      cavity_point(c, rc + 1, fixed, s[k], l[k], d[k], u[k], w[k], rhs);
+    if constexpr (LINE_X) f[k] = rhs;
+    if (r < 2) s[k] = 0.0;
+    if (r < 1) l[k] = 0.0;
+    if (r + 1 >= len) u[k] = 0.0;
+    if (r + 2 >= len) w[k] = 0.0;
+    if (r >= len) {
+      d[k] = 1.0;
+      if constexpr (LINE_X) f[k] = 0.0;
+    }
+  }
+}
+
+// x solve of every grid row i (system i - 1), L = 2^lshift lanes of M rows
+// each, 256 / L rows per workgroup; the solution goes to x[(i-1)*nx_pad + j-1]
+template <int M>
+__global__ void __launch_bounds__(256) k_cavity_x(Cavity c, double *__restrict__ x,
+                                                  int lshift) {
+  const int L = 1 << lshift;
+  const int sl = (int)threadIdx.x >> lshift;
+  const int p = (int)threadIdx.x & (L - 1);
+  const int64_t sys = ((int64_t)blockIdx.x << (8 - lshift)) + sl;
+  const int i = (int)(sys < c.ny ? sys : c.ny - 1) + 1;
+  WaveEx ex{(int)(threadIdx.x & 63), p, L};
+  double s[M], l[M], d[M], u[M], w[M], f[M];
+  cavity_rows<M, true>(c, i, p * M, s, l, d, u, w, f);
+  part_solve<double, M>(s, l, d, u, w, f, p, L, ex);
+  if (sys < c.ny) {
+    double *row = x + (int64_t)(i - 1) * c.nx_pad;
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      if (p * M + k < c.nx) row[p * M + k] = f[k];
+  }
+}
+
+// y solve of every grid column j (system j - 1): S columns x P chunks of M
+// rows per workgroup, thread t -> column t % S, chunk t / S
+template <int M>
+__global__ void __launch_bounds__(512) k_cavity_y(Cavity c, double *__restrict__ x, int S,
+                                                  int P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int t = threadIdx.x;
+  const int sl = t % S, p = t / S;
+  const int64_t sys = xcd_tile(blockIdx.x, gridDim.x) * S + sl;
+  const bool valid = sys < c.nx;
+  const int j = (int)(valid ? sys : c.nx - 1) + 1;
+  double *col = x + (j - 1);
+  double s[M], l[M], d[M], u[M], w[M], f[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int r = p * M + k;
+    f[k] = r < c.ny ? col[(int64_t)r * c.nx_pad] : 0.0;
+  }
+  cavity_rows<M, false>(c, j, p * M, s, l, d, u, w, f);
+  LdsEx ex{smem_raw, sl, p, S, P};
+  part_solve<double, M>(s, l, d, u, w, f, p, P, ex);
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int r = p * M + k;
+      if (r < c.ny) col[(int64_t)r * c.nx_pad] = f[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+bool cavity_plan_x(int nx, int *rows, int *lanes) {
+  for (int m : {8, 16}) {
+    int l = 1;
+    while ((int64_t)l * m < nx) l <<= 1;
+    if (l <= 64) {
+      *rows = m;
+      *lanes = l;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool cavity_plan_y(int ny, int *rows, int *sys_per_wg) {
+  for (int m : {8, 16}) {
+    const int P = (ny + m - 1) / m;
+    for (int S = 64; S >= 1; S >>= 1) {
+      if ((int64_t)S * P <= 512) {
+        *rows = m;
+        *sys_per_wg = S;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+hipError_t launch_cavity_assemble(const Cavity &c, const CavityOut &o, hipStream_t st) {
+  const int64_t n = (int64_t)c.nx * c.ny;
+  const int64_t grid = (n + 255) / 256;
+  hipLaunchKernelGGL(k_cavity_assemble, dim3((unsigned)grid), dim3(256), 0, st, c, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_cavity_solve(const Cavity &c, double *x, hipStream_t st) {
+  int mx, lanes, my, S;
+  if (!cavity_plan_x(c.nx, &mx, &lanes) || !cavity_plan_y(c.ny, &my, &S))
+    return hipErrorInvalidValue;
+  int lshift = 0;
+  while ((1 << lshift) < lanes) ++lshift;
+  const int64_t rows_per_wg = 256 >> lshift;
+  const int64_t gx = (c.ny + rows_per_wg - 1) / rows_per_wg;
+  if (mx == 8)
+    hipLaunchKernelGGL(k_cavity_x<8>, dim3((unsigned)gx), dim3(256), 0, st, c, x, lshift);
+  else
+    hipLaunchKernelGGL(k_cavity_x<16>, dim3((unsigned)gx), dim3(256), 0, st, c, x, lshift);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int P = (c.ny + my - 1) / my;
+  const size_t smem = (size_t)10 * S * P * sizeof(double);
+  const int64_t gy = (c.nx + S - 1) / S;
+  auto go = [&](auto kernel) {
+    if (smem > 65536) {
+      const hipError_t a = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)smem);
+      if (a != hipSuccess) return a;
+    }
+    hipLaunchKernelGGL(kernel, dim3((unsigned)gy), dim3(S * P), smem, st, c, x, S, P);
+    return hipGetLastError();
+  };
+  return my == 8 ? go(k_cavity_y<8>) : go(k_cavity_y<16>);
+}
+
+} // namespace pentad

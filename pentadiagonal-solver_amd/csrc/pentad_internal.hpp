@@ -202,4 +202,26 @@ hipError_t launch_ir_lane(const Diags<float> &a, double *x, const Lines &ln,
                           void *scratch, int max_steps, double tol, int *steps,
                           hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// 2-D cavity consumer: pressure-correction assembly and solve
+// (kernels_cavity.hip).  Inputs row-major with the reference's extents:
+// up = u' (ny+2) x (nx+1), vp = v' (ny+1) x (nx+2), uv (ny+2) x (nx+2);
+// the field and the arrays have rows of nx_pad elements.
+// ---------------------------------------------------------------------------
+struct Cavity {
+  int nx = 0, ny = 0, nx_pad = 0;
+  double alpha_u = 0, dx = 0, dy = 0;
+  const double *up = nullptr, *vp = nullptr, *uv = nullptr;
+};
+// x, then the x set (ds, dl, d, du, dw), then the y set (p[6] == nullptr:
+// no y set)
+struct CavityOut {
+  double *p[11] = {};
+};
+bool cavity_plan_x(int nx, int *rows, int *lanes);
+bool cavity_plan_y(int ny, int *rows, int *sys_per_wg);
+hipError_t launch_cavity_assemble(const Cavity &c, const CavityOut &o, hipStream_t st);
+// assembly fused into the x solve, then the y solve, in place on x
+hipError_t launch_cavity_solve(const Cavity &c, double *x, hipStream_t st);
+
 } // namespace pentad

@@ -1,0 +1,175 @@
+"""The 2-D cavity consumer's pressure-correction step on the GPU.
+
+Reference: apps/2D_cavity_cpu.cpp:195-219 -- assembly (apps/preproc.hpp:248-525,
+pressure_coef_matrix) into arrays with rows of nx_pad, then an x solve and a
+y solve of the padded field with the same values.
+
+  * Assembly (pentadsolver_D_cavity_pressure_coef): bit for bit equal to the
+    oracle's restatement (itself bit-identical to the reference's compiled
+    code, tests/test_cavity_oracle.py) at every point, also with nx_pad > nx;
+    the padding untouched; the y set equal to the x set.
+  * The whole step, two ways -- assembly + the padded batch solves, and the
+    fused solve (pentadsolver_D_cavity_pressure_solve, coefficients
+    generated inside the solve kernels) -- against the long-double oracle run
+    on the same assembled systems: max-norm relative forward error <= 1e-12.
+    These systems are only weakly diagonally dominant in the interior (d =
+    the sum of the four couplings; infinity-norm condition numbers about 1e3
+    for 16 points per line, 1e4 for 100, 5e5 for 512, 1.4e6 for 1024), yet
+    the measured errors stay below 1e-14; the normwise backward error of the
+    x solve is checked as well (<= 1e-14).  Where both ways run the same
+    partition, their results are bit-identical.
+  * With nx == nx_pad, the fused step against the reference's whole CPU
+    pipeline (its compiled assembly and its compiled AVX solver, oracle/_ref).
+"""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+import pentadsolver_amd as P
+from oracle import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_spec = importlib.util.spec_from_file_location(
+    "_workloads", os.path.join(HERE, "..", "pentadiagonal-solver_amd", "workloads.py"))
+W = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(W)
+
+SENT = -3.5
+SHAPES = [(16, 16, 16), (12, 9, 12), (14, 9, 16), (30, 7, 32), (3, 3, 4), (100, 37, 104),
+          (128, 128, 128), (513, 20, 516), (1024, 64, 1024)]
+XKEYS = ("s_x", "l_x", "d_x", "u_x", "w_x")
+YKEYS = ("s_y", "l_y", "d_y", "u_y", "w_y")
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def oracle_dense(nx, ny, inputs):
+    """The assembly with consistent indexing: the oracle at nx_pad = nx."""
+    up, vp, uv, aU, dx, dy = inputs
+    return O.cavity_pressure_coef(nx, ny, nx, aU, dx, dy, up, vp, uv)
+
+
+def tol(nx, ny):
+    return 1e-12
+
+
+def relerr(a, ref):
+    return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
+
+
+def ld_step(o, nx, ny):
+    """x solve then y solve of the assembled systems in long double."""
+    xs = O.solve_ld(*[o[k] for k in XKEYS], o["x"], [nx, ny], 0)
+    return xs, O.solve_ld(*[o[k] for k in YKEYS], xs, [nx, ny], 1)
+
+
+@pytest.mark.parametrize("nx,ny,nx_pad", SHAPES)
+def test_assembly_bit_identical(nx, ny, nx_pad):
+    print(f"cavity assembly nx={nx} ny={ny} nx_pad={nx_pad}", flush=True)
+    inputs = W.cavity_np(nx, ny, seed=nx + 7 * ny)
+    up, vp, uv, aU, dx, dy = inputs
+    dense = oracle_dense(nx, ny, inputs)
+    pads = [nx_pad, ny]
+    outs = {k: cuda(np.full(nx_pad * ny, SENT)) for k in O.CAVITY_OUT}
+    P.cavity_pressure_coef(nx, ny, nx_pad, aU, dx, dy, cuda(up), cuda(vp), cuda(uv), outs["x"],
+                           [outs[k] for k in XKEYS], [outs[k] for k in YKEYS])
+    torch.cuda.synchronize()
+    mask = O.padding_mask([nx, ny], pads)
+    for k in O.CAVITY_OUT:
+        got = outs[k].cpu().numpy()
+        assert np.array_equal(O.extract_padded(got, [nx, ny], pads), dense[k]), k
+        assert np.all(got[mask] == SENT), k
+    if nx == nx_pad and O.ref_cavity_available():
+        ref = O.ref_cavity_pressure_coef(nx, ny, nx, aU, dx, dy, up, vp, uv)
+        for k in O.CAVITY_OUT:
+            assert np.array_equal(outs[k].cpu().numpy(), ref[k]), k
+
+
+@pytest.mark.parametrize("nx,ny,nx_pad", SHAPES)
+def test_step_assembled_and_fused(nx, ny, nx_pad):
+    print(f"cavity step nx={nx} ny={ny} nx_pad={nx_pad}", flush=True)
+    inputs = W.cavity_np(nx, ny, seed=3 * nx + ny)
+    up, vp, uv, aU, dx, dy = inputs
+    dense = oracle_dense(nx, ny, inputs)
+    x_ld, ref = ld_step(dense, nx, ny)
+    dims, pads = [nx, ny], [nx_pad, ny]
+    g = [cuda(a) for a in (up, vp, uv)]
+    with P.Handle() as h:
+        # assembly + the padded batch solves (the reference's sequence)
+        outs = {k: cuda(np.full(nx_pad * ny, SENT)) for k in O.CAVITY_OUT}
+        P.cavity_pressure_coef(nx, ny, nx_pad, aU, dx, dy, *g, outs["x"],
+                               [outs[k] for k in XKEYS], [outs[k] for k in YKEYS], handle=h)
+        xa = outs["x"]
+        for sd, keys in ((0, XKEYS), (1, YKEYS)):
+            c = [outs[k] for k in keys]
+            n = P.gpsv_batch_padded_buffer_size(*c, xa, dims, pads, sd, h)
+            buf = torch.empty(n, dtype=torch.uint8, device="cuda") if n else None
+            P.gpsv_batch_padded(*c, xa, dims, pads, sd, buf, h)
+            if sd == 0:
+                torch.cuda.synchronize()
+                x_after_x = O.extract_padded(xa.cpu().numpy(), dims, pads)
+        cfg_a = (h.kernel_config(0), h.kernel_config(2))
+        # fused
+        xf = cuda(np.full(nx_pad * ny, SENT))
+        P.cavity_pressure_solve(nx, ny, nx_pad, aU, dx, dy, *g, xf, handle=h)
+        torch.cuda.synchronize()
+        cfg_f = (h.kernel_config(0), h.kernel_config(2))
+    a = O.extract_padded(xa.cpu().numpy(), dims, pads)
+    f_full = xf.cpu().numpy()
+    f = O.extract_padded(f_full, dims, pads)
+    ea, ef = relerr(a, ref), relerr(f, ref)
+    # normwise backward error of the GPU x solve on the assembled systems,
+    # |A x - b| / (|A| |x| + |b|) in the max norm (|A| <= 2 max d: d is the
+    # sum of the four couplings, or exceeds it)
+    rx = O.residual(*[dense[k] for k in XKEYS], x_after_x, dense["x"], dims, 0)
+    rx /= 2 * np.max(np.abs(dense["d_x"])) * np.max(np.abs(x_after_x)) + np.max(np.abs(dense["x"]))
+    print(f"  assembled {cfg_a} err {ea:.2e} (x-solve residual {rx:.1e}); fused {cfg_f} "
+          f"err {ef:.2e}; fused vs assembled {relerr(f, a):.2e}", flush=True)
+    assert ea < tol(nx, ny)
+    assert ef < tol(nx, ny)
+    assert rx < 1e-14
+    assert np.all(f_full[O.padding_mask(dims, pads)] == SENT)
+    assert cfg_f[0].startswith("cavity") and cfg_f[1].startswith("cavity")
+    if all(a.split()[1:] == b.split()[1:] for a, b in zip(cfg_a, cfg_f)):
+        assert np.array_equal(f, a)  # same partition: only the coefficient source differs
+
+
+@pytest.mark.skipif(not (O.ref_cavity_available() and O.ref_available("avx")),
+                    reason="oracle/_ref not built")
+@pytest.mark.parametrize("nx,ny", [(16, 16), (64, 40), (128, 96)])
+def test_fused_step_matches_reference_cpu_pipeline(nx, ny):
+    """nx == nx_pad (the reference consumer's own condition): its compiled
+    assembly, then its compiled AVX solver along x and along y."""
+    up, vp, uv, aU, dx, dy = W.cavity_np(nx, ny, seed=nx * ny)
+    r = O.ref_cavity_pressure_coef(nx, ny, nx, aU, dx, dy, up, vp, uv)
+    xr = r["x"].copy()
+    O.ref_gpsv_batch(*[r[k] for k in XKEYS], xr, [nx, ny], 0)
+    O.ref_gpsv_batch(*[r[k] for k in YKEYS], xr, [nx, ny], 1)
+    xf = cuda(np.zeros(nx * ny))
+    P.cavity_pressure_solve(nx, ny, nx, aU, dx, dy, cuda(up), cuda(vp), cuda(uv), xf)
+    torch.cuda.synchronize()
+    err = relerr(xf.cpu().numpy(), xr)
+    print(f"cavity fused vs reference CPU pipeline {nx}x{ny}: {err:.2e}", flush=True)
+    assert err < 1e-12
+
+
+def test_cavity_rejects_bad_arguments():
+    up, vp, uv = (torch.ones(4096, dtype=torch.float64, device="cuda") for _ in range(3))
+    x = torch.zeros(4096, dtype=torch.float64, device="cuda")
+    with pytest.raises(P.PentadError):
+        P.cavity_pressure_solve(2, 8, 2, 0.7, 0.5, 0.1, up, vp, uv, x)  # nx < 3
+    with pytest.raises(P.PentadError):
+        P.cavity_pressure_solve(8, 8, 6, 0.7, 0.1, 0.1, up, vp, uv, x)  # nx_pad < nx
+    big = torch.ones(1030 * 8 + 2 * 1030 + 16, dtype=torch.float64, device="cuda")
+    xb = torch.zeros(1030 * 8, dtype=torch.float64, device="cuda")
+    with pytest.raises(P.PentadError):
+        P.cavity_pressure_solve(1030, 4, 1030, 0.7, 1e-3, 0.25, big, big, big, xb)  # nx > 1024
+    with pytest.raises(ValueError):
+        P.cavity_pressure_solve(64, 64, 64, 0.7, 1e-2, 1e-2, up, vp, uv[:100], x)
