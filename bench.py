@@ -30,7 +30,10 @@ log-uniform length in [32, 8192] in one launch vs one call per system, and
 the 384x256x640 hyperdiffusion run (100 steps as one hipGraph, FP32+FP64
 refinement to --c5-tol vs FP64, the first --c5-cpu-steps steps also on the
 CPU; and the same operators as structured coefficients, generated in the
-kernels from one scale per line, FP64 and FP32).
+kernels from one scale per line, FP64 and FP32), and the 2-D cavity
+consumer's pressure-correction step (1024 x 1024 and 1024 x 8192, the
+reference's assembled arrays vs the fused solve; at 1024 x 1024 also the
+reference's CPU pipeline).
 --no-extra skips them.
 
 Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--size 512]
@@ -378,6 +381,87 @@ def workload_c4(P, torch, W, h, check, reps=10):
     return res
 
 
+def workload_cavity(P, torch, W, h, check, reps=20, cpu_reps=2):
+    """The 2-D cavity consumer's pressure-correction step (reference
+    apps/2D_cavity_cpu.cpp:195-219): assembly of the systems
+    (apps/preproc.hpp:248-525) + x solve + y solve of the field, rows of
+    nx_pad = nx.  Grids 1024 x 1024 (the consumer's N_MAX) and 1024 x 8192;
+    rows per step = 2 nx ny.  Two ways on the GPU: the reference's sequence
+    (assembly kernel writing the right-hand side and both sets of five
+    diagonals, then the padded batch solves: 224 B per point) and the fused
+    solve (coefficients generated inside the solve kernels: 56 B per point).
+    At 1024 x 1024 also the reference's own CPU pipeline (its compiled
+    assembly and AVX solver, oracle/_ref) on the same inputs."""
+    from oracle import oracle as O
+    stream = torch.cuda.current_stream()
+    res = {"bytes_per_point": {"assembled": 224, "fused": 56}}
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    for nx, ny in ((1024, 1024), (1024, 8192)):
+        up, vp, uv, aU, dx, dy = W.cavity_torch(nx, ny, seed=11)
+        dims = pads = [nx, ny]
+        n = nx * ny
+        arr = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(11)]
+        x, xset, yset = arr[0], arr[1:6], arr[6:11]
+        bufs = []
+        for sd, cs in ((0, xset), (1, yset)):
+            nb = P.gpsv_batch_padded_buffer_size(*cs, x, dims, pads, sd, h)
+            bufs.append(torch.empty(nb, dtype=torch.uint8, device="cuda") if nb else None)
+
+        def assembled():
+            P.cavity_pressure_coef(nx, ny, nx, aU, dx, dy, up, vp, uv, x, xset, yset, handle=h)
+            P.gpsv_batch_padded(*xset, x, dims, pads, 0, bufs[0], h)
+            P.gpsv_batch_padded(*yset, x, dims, pads, 1, bufs[1], h)
+
+        xf = torch.empty(n, dtype=torch.float64, device="cuda")
+
+        def fused():
+            P.cavity_pressure_solve(nx, ny, nx, aU, dx, dy, up, vp, uv, xf, handle=h)
+
+        ms_a = timed(assembled, reps)
+        cfg_a = [h.kernel_config(0), h.kernel_config(2)]
+        ms_f = timed(fused, reps)
+        cfg_f = [h.kernel_config(0), h.kernel_config(2)]
+        rows = 2 * n
+        r = {"grid": [nx, ny],
+             "assembled": {"ms_per_step": ms_a, "grows": rows / (ms_a * 1e-3) / 1e9,
+                           "gbs": 224 * n / (ms_a * 1e-3) / 1e9, "kernels": cfg_a},
+             "fused": {"ms_per_step": ms_f, "grows": rows / (ms_f * 1e-3) / 1e9,
+                       "gbs": 56 * n / (ms_f * 1e-3) / 1e9, "kernels": cfg_f},
+             "fused_speedup": ms_a / ms_f,
+             "max_rel_diff_fused_vs_assembled": max_rel(xf.cpu().numpy(), x.cpu().numpy())}
+        if check and nx * ny <= 1 << 20 and O.ref_cavity_available() and O.ref_available("avx"):
+            npin = [t.cpu().numpy() for t in (up, vp, uv)]
+            t0 = time.perf_counter()
+            for _ in range(cpu_reps):
+                ro = O.ref_cavity_pressure_coef(nx, ny, nx, aU, dx, dy, *npin)
+                xr = ro["x"]
+                O.ref_gpsv_batch(*[ro[k] for k in ("s_x", "l_x", "d_x", "u_x", "w_x")], xr,
+                                 dims, 0)
+                O.ref_gpsv_batch(*[ro[k] for k in ("s_y", "l_y", "d_y", "u_y", "w_y")], xr,
+                                 dims, 1)
+            ms_c = (time.perf_counter() - t0) * 1e3 / cpu_reps
+            r["reference_cpu"] = {"ms_per_step": ms_c, "grows": rows / (ms_c * 1e-3) / 1e9,
+                                  "threads": os.environ.get("OMP_NUM_THREADS", "all"),
+                                  "note": "compiled assembly (inputs converted to its nested "
+                                          "vectors inside the timed region) + AVX solver x, y"}
+            r["max_rel_err_fused_vs_reference_cpu"] = max_rel(xf.cpu().numpy(), xr)
+        res[f"{nx}x{ny}"] = r
+        del arr, x, xset, yset, xf, up, vp, uv, bufs
+        torch.cuda.empty_cache()
+    return res
+
+
 def workload_c5(P, torch, W, h, check, steps=100, cpu_steps=2, max_refine=5, tol=1e-13):
     """Hyperdiffusion config: 384x256x640, I + nu dt/h^4 [1,-4,6,-4,1] per
     direction with nu ~ U(0.5, 2) per grid line (seed 7), N(0,1) initial
@@ -632,6 +716,8 @@ def main():
         wl["c5_hyperdiffusion"] = workload_c5(P, torch, W, h, chk, cpu_steps=args.c5_cpu_steps,
                                               max_refine=args.max_refine,
                                               tol=args.c5_tol)
+        log("bench: 2-D cavity consumer pressure step, assembled vs fused")
+        wl["cavity_pressure_step"] = workload_cavity(P, torch, W, h, chk)
         result["workloads"] = wl
     h.close()
     print(json.dumps(result), flush=True)

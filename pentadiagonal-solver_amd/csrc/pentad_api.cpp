@@ -286,6 +286,21 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
         }
       }
     }
+    // long lines (no 16 systems of them fit one workgroup): the widest
+    // workgroup that fits, before the one-lane kernel with its scratch
+    // passes
+    for (int S = 8; S >= 1; S >>= 1) {
+      for (int k = 0; k < 10; ++k) {
+        const int r = pref[k % 5];
+        if ((k < 5) != (ln.len % r == 0)) continue;
+        if (part_yz_supported(elem_bytes, r, S, ln.len)) {
+          pl.kind = Plan::PartYZ;
+          pl.rows = r;
+          pl.sys_per_wg = S;
+          return pl;
+        }
+      }
+    }
   }
   pl.kind = Plan::Naive;
   return pl;

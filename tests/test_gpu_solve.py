@@ -228,6 +228,21 @@ def test_partition_yz_padded_lengths(n, sd, dtype):
         assert h.kernel_config(sd).startswith("thomas-pcr"), h.kernel_config(sd)
 
 
+@pytest.mark.parametrize("n", [1024, 1000, 2048, 8192])
+@pytest.mark.parametrize("sd", [1, 2])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_partition_yz_long_lines(n, sd, dtype):
+    """Lines too long for 16 systems per workgroup run the partitioned
+    kernel with fewer systems per workgroup, not the one-lane fallback."""
+    dims = [20, n, 2] if sd == 1 else [20, 2, n]
+    print(f"yz long len={n} sd={sd} {np.dtype(dtype).name}", flush=True)
+    with P.Handle() as h:
+        check(gen(dims, n + 3 * sd, dtype), dims, sd, h)
+        cfg = h.kernel_config(sd)
+        print(f"  {cfg}", flush=True)
+        assert cfg.startswith("thomas-pcr"), cfg
+
+
 def test_streams_and_handles():
     dims = [256, 32, 16]
     arrs = gen(dims, 3)
