@@ -40,3 +40,21 @@ FIXTURES = os.path.join(GOLDEN, "reference_fixtures")
 FIXTURE_NAMES = ["one_dim_small_solve0", "one_dim_large_solve0",
                  "two_dim_small_solve0", "two_dim_small_solve1",
                  "two_dim_large_solve0", "two_dim_large_solve1"]
+
+
+# The reference's three 3-D fixtures (three_dim_large_solve{0,1,2}, 32^3) are
+# missing from its checkout; tests/golden/gen_fixtures.py restates its
+# generator and writes them (tests/test_fixture_generator.py pins the
+# generator against the six shipped fixtures).
+GENERATED_3D = ["three_dim_large_solve0", "three_dim_large_solve1", "three_dim_large_solve2"]
+
+
+@pytest.fixture(scope="session")
+def generated_fixtures(tmp_path_factory):
+    spec = importlib.util.spec_from_file_location(
+        "gen_fixtures", os.path.join(GOLDEN, "gen_fixtures.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    out = str(tmp_path_factory.mktemp("generated_fixtures"))
+    gen.generate(out)
+    return out

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import pentadsolver_amd as P
-from conftest import FIXTURES, FIXTURE_NAMES
+from conftest import FIXTURES, FIXTURE_NAMES, GENERATED_3D
 from oracle import oracle as O
 
 torch = pytest.importorskip("torch")
@@ -75,6 +75,37 @@ def test_reference_fixtures_on_gpu(name, dtype, algo):
         arrs = [fx[k] for k in ("ds", "dl", "d", "du", "dw", "x")]
         x = gpu_solve(arrs, fx["dims"], fx["solvedim"], h)
     assert O.allclose_reference_rule(fx["u"], x, dtype), name
+
+
+@pytest.mark.parametrize("name", GENERATED_3D)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("algo", ["auto", "naive", "thomas-pcr", "shared-fact"])
+def test_generated_3d_fixtures_on_gpu(name, dtype, algo, generated_fixtures):
+    """The reference's missing 32^3 fixtures, rewritten by its generator's
+    algorithm (tests/golden/gen_fixtures.py), every solve dimension.
+    Shared-fact solves every system with system 0's coefficients (the
+    reference's contract), so it is held to the oracle of that system."""
+    fx = O.read_fixture(os.path.join(generated_fixtures, name), dtype)
+    print(f"generated fixture {name} {np.dtype(dtype).name} {algo}", flush=True)
+    arrs = [fx[k] for k in ("ds", "dl", "d", "du", "dw", "x")]
+    with P.Handle() as h:
+        for k in range(3):
+            h.set_algo(k, algo)
+        x = gpu_solve(arrs, fx["dims"], fx["solvedim"], h)
+        cfg = h.kernel_config(fx["solvedim"])
+    if algo == "shared-fact":
+        ln = fx["dims"][fx["solvedim"]]
+        sysc = []
+        for a in arrs[:5]:
+            v = np.moveaxis(a.reshape(list(reversed(fx["dims"]))), 2 - fx["solvedim"], -1)
+            line0 = v.reshape(-1, ln)[0]
+            sysc.append(np.ascontiguousarray(
+                np.moveaxis(np.broadcast_to(line0, v.shape), -1, 2 - fx["solvedim"])).reshape(-1))
+        ref = O.solve(*[c.astype(np.float64) for c in sysc], arrs[5].astype(np.float64),
+                      fx["dims"], fx["solvedim"])
+        assert relerr(x, ref) < (1e-12 if dtype == np.float64 else 2e-5), cfg
+    else:
+        assert O.allclose_reference_rule(fx["u"], x, dtype), (name, cfg)
 
 
 SHAPES = [
