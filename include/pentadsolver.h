@@ -374,9 +374,11 @@ const char *pentadsolver_kernel_config(pentadsolver_handle_t handle, int dir);
  * builds a device schedule once (systems sorted longest first and grouped
  * into work items whose lanes per system follow the length); each solve is
  * then one persistent launch on the handle's stream with no allocation and
- * no host synchronisation (capturable in a hipGraph).  The plan keeps the
- * handle, which must outlive it.  No reference counterpart (the reference
- * batches systems of one length only). */
+ * no host synchronisation (capturable in a hipGraph).  The workgroups take
+ * work items from a counter held in the plan, so two solves of one plan
+ * must not run at the same time (use one plan per concurrent stream).  The
+ * plan keeps the handle, which must outlive it.  No reference counterpart
+ * (the reference batches systems of one length only). */
 typedef void *pentadsolver_varlen_t;
 pentadStatus_t pentadsolver_varlen_create(pentadsolver_handle_t handle,
                                           pentadsolver_varlen_t *plan,

@@ -39,7 +39,6 @@ namespace pentad {
 hipError_t launch_permute3(const void *in, int in_bytes, void *out, int out_bytes,
                            const int64_t dims[3], const int64_t ostride[3],
                            hipStream_t st);
-hipError_t launch_fill_int(int *p, int v, hipStream_t st);
 
 pentadStatus_t gpsv_batch_ir(Context *c, const float *ds, const float *dl,
                              const float *d, const float *du, const float *dw,

@@ -129,17 +129,24 @@ template <typename T>
 hipError_t launch_tr_x(const Diags<T> &a, T *x, const Lines &ln, int rows, int lanes,
                        hipStream_t st);
 
+// Writes one int with a kernel (usable inside captured graphs, where a
+// memset node is avoided).
+hipError_t launch_fill_int(int *p, int v, hipStream_t st);
+
 // Variable-length batch: systems packed back to back (offsets), solved in
 // one persistent launch over work items of S systems x P chunks of
-// VL_ROWS rows (S * P = VL_BLOCK); lengths 3 .. VL_BLOCK * VL_ROWS.
+// VL_ROWS rows (S * P = VL_BLOCK); lengths 3 .. VL_BLOCK * VL_ROWS.  The
+// workgroups take items in list order from the device counter `next`,
+// which the launcher resets first (two solves sharing one counter must not
+// overlap).
 constexpr int VL_BLOCK = 512;
 constexpr int VL_ROWS = 16;
 template <typename T>
 hipError_t launch_varlen(const Diags<T> &a, T *x, const int64_t *offs, const int *perm,
-                         const int2 *items, int n_items, hipStream_t st);
+                         const int2 *items, int n_items, int *next, hipStream_t st);
 hipError_t launch_varlen_ir(const Diags<float> &a, double *x, const int64_t *offs,
-                            const int *perm, const int2 *items, int n_items, int max_steps,
-                            double tol, int *steps, hipStream_t st);
+                            const int *perm, const int2 *items, int n_items, int *next,
+                            int max_steps, double tol, int *steps, hipStream_t st);
 
 // Shared factorisation (every system of the direction has system 0's
 // coefficients): k_sf_factor writes the factor table of system 0 into `tab`

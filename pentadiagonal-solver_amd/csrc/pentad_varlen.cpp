@@ -2,8 +2,10 @@
 //
 // pentadsolver_varlen_create turns the caller's offsets into a device
 // schedule once (sorted permutation + work items, see kernels_varlen.hip);
-// the solves then launch one persistent kernel each, with no host work and
-// no allocation, so they can be captured in a hipGraph.
+// the solves then launch one persistent kernel each (after a one-int fill
+// of the plan's work counter), with no host work and no allocation, so they
+// can be captured in a hipGraph.  Solves of one plan share that counter and
+// must not overlap (one stream per plan).
 // No reference counterpart (the reference solves batches of equal length).
 #include "pentad_api_internal.hpp"
 
@@ -21,6 +23,7 @@ struct Varlen {
   int64_t *offs = nullptr; // device, n_sys + 1
   int *perm = nullptr;     // device, n_sys
   int2 *items = nullptr;   // device, n_items
+  int *next = nullptr;     // device, the solves' work counter
   void *mem = nullptr;
 };
 
@@ -67,13 +70,14 @@ pentadStatus_t pentadsolver_varlen_create(pentadsolver_handle_t handle,
   const size_t b_off = (size_t)(n_sys + 1) * sizeof(int64_t);
   const size_t b_perm = ((size_t)n_sys * sizeof(int) + 15) & ~(size_t)15;
   const size_t b_items = items.size() * sizeof(int2);
-  if (hipMalloc(&V->mem, b_off + b_perm + b_items) != hipSuccess) {
+  if (hipMalloc(&V->mem, b_off + b_perm + b_items + sizeof(int)) != hipSuccess) {
     delete V;
     return PENTAD_STATUS_ALLOC_FAILED;
   }
   V->offs = static_cast<int64_t *>(V->mem);
   V->perm = reinterpret_cast<int *>(static_cast<char *>(V->mem) + b_off);
   V->items = reinterpret_cast<int2 *>(static_cast<char *>(V->mem) + b_off + b_perm);
+  V->next = reinterpret_cast<int *>(static_cast<char *>(V->mem) + b_off + b_perm + b_items);
   if (hipMemcpy(V->offs, offsets, b_off, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(V->perm, perm.data(), (size_t)n_sys * sizeof(int), hipMemcpyHostToDevice) !=
           hipSuccess ||
@@ -120,7 +124,7 @@ pentadStatus_t pentadsolver_D_gpsv_varlen_batch(pentadsolver_varlen_t plan, cons
   if (!V || !ds || !dl || !d || !du || !dw || !x) return PENTAD_STATUS_INVALID_VALUE;
   return finish(V,
                 launch_varlen<double>(Diags<double>{ds, dl, d, du, dw}, x, V->offs, V->perm,
-                                      V->items, V->n_items, V->c->stream),
+                                      V->items, V->n_items, V->next, V->c->stream),
                 "varlen FP64");
 }
 
@@ -131,7 +135,7 @@ pentadStatus_t pentadsolver_S_gpsv_varlen_batch(pentadsolver_varlen_t plan, cons
   if (!V || !ds || !dl || !d || !du || !dw || !x) return PENTAD_STATUS_INVALID_VALUE;
   return finish(V,
                 launch_varlen<float>(Diags<float>{ds, dl, d, du, dw}, x, V->offs, V->perm,
-                                     V->items, V->n_items, V->c->stream),
+                                     V->items, V->n_items, V->next, V->c->stream),
                 "varlen FP32");
 }
 
@@ -146,8 +150,8 @@ pentadStatus_t pentadsolver_DS_gpsv_varlen_batch_ir(pentadsolver_varlen_t plan,
     return PENTAD_STATUS_INVALID_VALUE;
   return finish(V,
                 launch_varlen_ir(Diags<float>{ds, dl, d, du, dw}, x, V->offs, V->perm,
-                                 V->items, V->n_items, max_refine, tol, d_refine_steps,
-                                 V->c->stream),
+                                 V->items, V->n_items, V->next, max_refine, tol,
+                                 d_refine_steps, V->c->stream),
                 "varlen FP32+FP64 refinement");
 }
 
