@@ -135,8 +135,22 @@ def test_naive_kernel_matches_oracle(dims, sd, dtype):
 # every partitioned instantiation, x: (rows, lanes) with rows*lanes = len
 X_CONFIGS = {
     np.float64: [(4, 16), (4, 64), (6, 32), (8, 8), (8, 32), (8, 64), (10, 32), (12, 16), (16, 16), (16, 32)],
-    np.float32: [(4, 16), (4, 64), (8, 8), (8, 32), (8, 64), (12, 16), (16, 16), (16, 32)],
+    np.float32: [(4, 16), (4, 64), (6, 32), (8, 8), (8, 32), (8, 64), (10, 32), (10, 64),
+                 (12, 16), (16, 16), (16, 32)],
 }
+
+
+@pytest.mark.parametrize("n", [640, 320, 160, 384, 96])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_x_lengths_with_6_or_10_row_chunks(n, dtype):
+    """Lengths 10 * 2^k and 6 * 2^k take the partitioned kernel in both
+    precisions (not the LDS-staged one)."""
+    dims = [n, 21, 4]
+    print(f"x auto len={n} {np.dtype(dtype).name}", flush=True)
+    with P.Handle() as h:
+        check(gen(dims, n + 1, dtype), dims, 0, h)
+        cfg = h.kernel_config(0)
+        assert cfg.startswith("thomas-pcr"), cfg
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
