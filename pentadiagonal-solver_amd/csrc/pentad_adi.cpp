@@ -355,7 +355,7 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
       }
       continue;
     }
-    A->plan[d] = (mode == 2) ? plan_ir(c, ln, aligned) : plan_solve(c, ln, A->cbytes, aligned);
+    A->plan[d] = (mode == 2) ? plan_ir(c, ln, aligned) : plan_solve(c, ln, A->cbytes, aligned, false);
     if (A->plan[d].kind != Plan::PartX && A->plan[d].kind != Plan::IrX) A->rotating = false;
   }
   // the caller's x must be 16-byte aligned for the vector loads of the first

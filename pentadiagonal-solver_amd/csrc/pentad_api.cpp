@@ -146,7 +146,7 @@ static bool x_aligned(const Lines &ln, int elem_bytes, const void *const ptrs[6]
 }
 
 Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
-                const void *const ptrs[6]) {
+                const void *const ptrs[6], bool in_place) {
   Plan pl;
   const std::string want = requested_algo(c, ln.dir);
   const bool is64 = (elem_bytes == 8);
@@ -248,6 +248,9 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
       const int npref = 6;
       for (int k = 0; k < npref; ++k) {
         const int r = pref[k];
+        // FP64 in place, 16 rows x 64 lanes (1024 rows): the LDS-staged
+        // kernel is faster (measured 0.22 against 0.32 ms for 2^23 rows)
+        if (is64 && r == 16 && ln.len == 1024 && in_place && ln.dense()) continue;
         if (ln.len % r) continue;
         const int64_t lanes = ln.len / r;
         if (lanes > 64 || lanes < 2 || (lanes & (lanes - 1))) continue;
@@ -271,6 +274,34 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
       pl.rows = tuned.rows;
       pl.sys_per_wg = tuned.sys_per_wg;
       return pl;
+    }
+    // lines of 32..384 rows: large chunks -- 16 or 12 rows, whichever
+    // splits the line into a power-of-two number of whole chunks, else
+    // FP64 16 rows, FP32 16 rows up to 128 and 12 above (its kernel runs
+    // 1024-thread workgroups) -- and 32 systems per workgroup, 16 when
+    // that leaves fewer than 2048 workgroups.  Measured with 2^26 rows
+    // along z, against the whole-chunks-first choice below: FP64 256 rows
+    // 0.86 vs 1.22 ms, 128 rows 0.79 vs 0.93, 100 rows 0.81 vs 0.93; FP32
+    // 384 rows 0.41 vs 0.49; within 5% of the best configuration the
+    // autotuner finds at the lengths measured (40 .. 384).
+    if (ln.len >= 32 && ln.len <= 384) {
+      auto pow2_chunks = [&](int r) {
+        const int64_t P = ln.len / r;
+        return ln.len % r == 0 && (P & (P - 1)) == 0;
+      };
+      const int r = pow2_chunks(16)   ? 16
+                    : pow2_chunks(12) ? 12
+                    : (is64 || ln.len <= 128) ? 16 : 12;
+      int S = 32;
+      while (S > 8 && !part_yz_supported(elem_bytes, r, S, ln.len)) S >>= 1;
+      if (S > 16 && ln.n_sys / S < 2048) S = 16;
+      if (is64 && ln.dir == 1 && S > 16) S = 16; // middle axis: 0.69 vs 0.81 ms, 192 rows
+      if (part_yz_supported(elem_bytes, r, S, ln.len)) {
+        pl.kind = Plan::PartYZ;
+        pl.rows = r;
+        pl.sys_per_wg = S;
+        return pl;
+      }
     }
     static const int pref64[] = {8, 6, 12, 4, 16};
     static const int pref32[] = {16, 12, 8, 6, 4};
