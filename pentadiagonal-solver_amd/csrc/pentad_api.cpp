@@ -401,7 +401,15 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
     pl.sys_per_wg = fwidth;
     return pl;
   }
-  static const int pref[] = {8, 6, 4, 10};
+  // 6-row chunks when they, and not 8-row ones, give a power-of-two chunk
+  // count (measured with 2^26 rows: 384 rows along z 1.43 vs 1.66 ms, 192
+  // rows 1.14 vs 1.39 ms)
+  auto pow2_chunks = [&](int r) {
+    const int64_t P = ln.len / r;
+    return ln.len % r == 0 && (P & (P - 1)) == 0;
+  };
+  static const int pref8[] = {8, 6, 4, 10}, pref6[] = {6, 8, 4, 10};
+  const int *pref = (!pow2_chunks(8) && pow2_chunks(6)) ? pref6 : pref8;
   for (int k = 0; k < 8; ++k) { // whole chunks first, then a padded last chunk
     const int r = pref[k % 4];
     if ((k < 4) != (ln.len % r == 0)) continue;
