@@ -214,7 +214,14 @@ pentadStatus_t pentadsolver_set_kernel_params(pentadsolver_handle_t handle,
  * out per row.  d_refine_steps (device int*, may be NULL) receives
  * atomicMax(*d_refine_steps, steps taken); the caller zeroes it.  t_buffer
  * as for pentadsolver_D_gpsv_batch, sized by the _ir_buffer_size_ext call.
- * No reference counterpart (the reference solves in one precision). */
+ * Along y / z the automatic choice is one FP64 solve of the coefficients
+ * widened on load instead: the same 20 + 8 + 8 B per row, the widened
+ * system's solution to FP64 rounding in one pass (what the refinement
+ * converges to), no refinement sweep (steps 0, tol and max_refine unused),
+ * because every sweep of the strided refinement kernel exchanges through
+ * LDS; pentadsolver_set_algo(h, dir, "thomas-pcr") keeps the refinement
+ * kernel there.  No reference counterpart (the reference solves in one
+ * precision). */
 size_t pentadsolver_DS_gpsv_batch_ir_buffer_size_ext(
     pentadsolver_handle_t handle, const float *ds, const float *dl,
     const float *d, const float *du, const float *dw, const double *x,

@@ -156,11 +156,14 @@ __global__ void __launch_bounds__(256, 2)
 // ---------------------------------------------------------------------------
 // strided (y/z) directions
 // ---------------------------------------------------------------------------
-template <typename T, int M, int MAXT>
+// C: the coefficients' storage type (float coefficients widened to a double
+// solve: the mixed-precision contract met by one FP64 solve, see
+// launch_part_yz_widened)
+template <typename T, int M, int MAXT, typename C = T>
 __global__ void __launch_bounds__(MAXT)
-    k_part_yz(const T *__restrict__ ds, const T *__restrict__ dl,
-              const T *__restrict__ d, const T *__restrict__ du,
-              const T *__restrict__ dw, T *__restrict__ x, Lines ln, int S,
+    k_part_yz(const C *__restrict__ ds, const C *__restrict__ dl,
+              const C *__restrict__ d, const C *__restrict__ du,
+              const C *__restrict__ dw, T *__restrict__ x, Lines ln, int S,
               int P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int t = threadIdx.x;
@@ -178,11 +181,11 @@ __global__ void __launch_bounds__(MAXT)
 #pragma unroll
     for (int i = 0; i < M; ++i) {
       const int64_t e = b + i * st;
-      s[i] = ds[e];
-      l[i] = dl[e];
-      dd[i] = d[e];
-      u[i] = du[e];
-      w[i] = dw[e];
+      s[i] = (T)ds[e];
+      l[i] = (T)dl[e];
+      dd[i] = (T)d[e];
+      u[i] = (T)du[e];
+      w[i] = (T)dw[e];
       f[i] = x[e];
     }
   } else {
@@ -193,11 +196,11 @@ __global__ void __launch_bounds__(MAXT)
       const int r = p * M + i;
       const int64_t e = b + i * st;
       const bool in = r < len;
-      s[i] = in ? ds[e] : T(0);
-      l[i] = in ? dl[e] : T(0);
-      dd[i] = in ? d[e] : T(1);
-      u[i] = (r + 1 < len) ? du[e] : T(0);
-      w[i] = (r + 2 < len) ? dw[e] : T(0);
+      s[i] = in ? (T)ds[e] : T(0);
+      l[i] = in ? (T)dl[e] : T(0);
+      dd[i] = in ? (T)d[e] : T(1);
+      u[i] = (r + 1 < len) ? (T)du[e] : T(0);
+      w[i] = (r + 2 < len) ? (T)dw[e] : T(0);
       f[i] = in ? x[e] : T(0);
     }
   }
@@ -378,8 +381,8 @@ template <typename T, int M> constexpr int yz_maxt() {
   return yz_maxt_rt((int)sizeof(T), M);
 }
 
-template <typename T, int M>
-hipError_t go_yz(const Diags<T> &a, T *x, const Lines &ln, int S,
+template <typename T, int M, typename C = T>
+hipError_t go_yz(const Diags<C> &a, T *x, const Lines &ln, int S,
                  hipStream_t st) {
   constexpr int MAXT = yz_maxt<T, M>();
   const int P = (int)((ln.len + M - 1) / M);
@@ -389,11 +392,11 @@ hipError_t go_yz(const Diags<T> &a, T *x, const Lines &ln, int S,
   const int64_t grid = (ln.n_sys + S - 1) / S;
   if (smem > 65536) {
     const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&k_part_yz<T, M, MAXT>),
+        reinterpret_cast<const void *>(&k_part_yz<T, M, MAXT, C>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_part_yz<T, M, MAXT>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((k_part_yz<T, M, MAXT, C>), dim3((unsigned)grid),
                      dim3(block), smem, st, a.ds, a.dl, a.d, a.du, a.dw, x, ln,
                      S, P);
   return hipGetLastError();
@@ -482,6 +485,19 @@ hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
   case 8: return go_yz<T, 8>(a, x, ln, sys_per_wg, st);
   case 12: return go_yz<T, 12>(a, x, ln, sys_per_wg, st);
   case 16: return go_yz<T, 16>(a, x, ln, sys_per_wg, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_part_yz_widened(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                                  int sys_per_wg, hipStream_t st) {
+  if (!part_yz_supported(8, rows, sys_per_wg, ln.len)) return hipErrorInvalidValue;
+  switch (rows) {
+  case 4: return go_yz<double, 4, float>(a, x, ln, sys_per_wg, st);
+  case 6: return go_yz<double, 6, float>(a, x, ln, sys_per_wg, st);
+  case 8: return go_yz<double, 8, float>(a, x, ln, sys_per_wg, st);
+  case 12: return go_yz<double, 12, float>(a, x, ln, sys_per_wg, st);
+  case 16: return go_yz<double, 16, float>(a, x, ln, sys_per_wg, st);
   }
   return hipErrorInvalidValue;
 }

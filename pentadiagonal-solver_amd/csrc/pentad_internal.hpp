@@ -114,6 +114,11 @@ hipError_t launch_part_x_map(const Diags<T> &a, const T *b, T *out, bool rot,
                              int64_t n_sys, int64_t len, const XMap &map, int rows,
                              int lanes, hipStream_t st);
 
+// The strided partitioned solve in FP64 arithmetic over FP32-stored
+// coefficients (widened on load): FP32 coefficients, FP64 right-hand side and
+// solution, the result of the widened system to FP64 rounding.
+hipError_t launch_part_yz_widened(const Diags<float> &a, double *x, const Lines &ln, int rows,
+                                  int sys_per_wg, hipStream_t st);
 // Partitioned solve, strided (y/z) directions: a workgroup owns `sys_per_wg`
 // adjacent systems and splits each into len/rows chunks, one lane per
 // (system, chunk); the reduced system is solved through LDS.

@@ -77,6 +77,17 @@ std::vector<Plan> candidates(const Lines &ln, int elem_bytes, bool ir,
       }
     }
   }
+  if (ir && ln.dir != 0) { // the FP64 solve of the widened coefficients
+    for (int r : {4, 6, 8, 12, 16})
+      for (int S = 4; S <= 64; S <<= 1)
+        if (part_yz_supported(8, r, S, ln.len)) {
+          Plan p;
+          p.kind = Plan::IrWideYZ;
+          p.rows = r;
+          p.sys_per_wg = S;
+          out.push_back(p);
+        }
+  }
   if (!ir && ln.dir == 0) { // LDS-staged solve ("transpose"), any length
     for (int r : {8, 16}) {
       int lanes = 1;

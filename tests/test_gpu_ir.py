@@ -36,6 +36,12 @@ def run_ir(a32, b, dims, sd, handle=None, max_refine=5, tol=1e-13):
     return x.cpu().numpy(), int(steps.item())
 
 
+def steps_ok(cfg, steps, lo=1, hi=5):
+    """The FP64 solve of the widened coefficients (strided lines, automatic
+    choice) takes no refinement sweep; the refinement kernels take 1..5."""
+    return steps == 0 if cfg.startswith("ir widened") else lo <= steps <= hi
+
+
 def reference(a32, b, dims, sd):
     a64 = [a.astype(np.float64) for a in a32]
     return O.solve(*a64, b.copy(), dims, sd)
@@ -57,11 +63,14 @@ def test_ir_matches_fp64_oracle(dims, sd, cond):
     ratio = (2.0, 3.0) if cond == "well" else (1.05, 1.05)
     a32, b = gen32(dims, 3 if cond == "well" else 4, ratio)
     print(f"ir dims={dims} sd={sd} {cond}", flush=True)
-    x, steps = run_ir(a32, b, dims, sd)
+    with P.Handle() as h:
+        x, steps = run_ir(a32, b, dims, sd, handle=h)
+        cfg = h.kernel_config(sd)
     err = relerr(x, reference(a32, b, dims, sd))
-    print(f"  err={err:.2e} steps={steps}", flush=True)
+    print(f"  err={err:.2e} steps={steps} {cfg}", flush=True)
     assert err < TOL
-    assert 1 <= steps <= 5
+    assert steps_ok(cfg, steps), (cfg, steps)
+    assert cfg.startswith("ir widened") == (sd != 0)
 
 
 def test_ir_fixed_steps_when_tol_zero():
@@ -72,13 +81,42 @@ def test_ir_fixed_steps_when_tol_zero():
     assert relerr(x, reference(a32, b, dims, 0)) < TOL
 
 
-def test_ir_zero_steps_is_plain_fp32():
-    dims = [256, 16, 4]
+@pytest.mark.parametrize("sd,algo", [(0, "auto"), (1, "thomas-pcr"), (2, "thomas-pcr")])
+def test_ir_zero_steps_is_plain_fp32(sd, algo):
+    """The refinement kernels with no sweep return the FP32 solve."""
+    dims = [256, 16, 4] if sd == 0 else ([16, 256, 4] if sd == 1 else [16, 4, 256])
     a32, b = gen32(dims, 6)
-    x, steps = run_ir(a32, b, dims, 1, max_refine=0)
-    err = relerr(x, reference(a32, b, dims, 1))
+    with P.Handle() as h:
+        h.set_algo(sd, algo)
+        x, steps = run_ir(a32, b, dims, sd, max_refine=0, handle=h)
+        assert h.kernel_config(sd).startswith("ir thomas-pcr"), h.kernel_config(sd)
+    err = relerr(x, reference(a32, b, dims, sd))
     assert steps == 0
     assert 1e-12 < err < 2e-5  # FP32 accuracy only, refinement needed
+
+
+@pytest.mark.parametrize("sd", [1, 2])
+def test_ir_strided_auto_is_one_fp64_solve(sd):
+    """Along y / z the automatic choice solves the widened coefficients in
+    FP64: FP64 accuracy with no refinement sweep, whatever max_refine says;
+    asking for "thomas-pcr" by name gives the refinement kernel, with the
+    same result to FP64 rounding."""
+    dims = [16, 512, 8] if sd == 1 else [16, 8, 512]
+    a32, b = gen32(dims, 9)
+    ref = reference(a32, b, dims, sd)
+    with P.Handle() as h:
+        x0, s0 = run_ir(a32, b, dims, sd, max_refine=0, handle=h)
+        cfg = h.kernel_config(sd)
+        h.set_algo(sd, "thomas-pcr")
+        x1, s1 = run_ir(a32, b, dims, sd, handle=h)
+        cfg1 = h.kernel_config(sd)
+    print(f"ir strided {cfg}: {relerr(x0, ref):.2e} | {cfg1}: {relerr(x1, ref):.2e} steps {s1}",
+          flush=True)
+    assert cfg.startswith("ir widened") and s0 == 0
+    assert relerr(x0, ref) < TOL
+    assert cfg1.startswith("ir thomas-pcr") and 1 <= s1 <= 5
+    assert relerr(x1, ref) < TOL
+    assert relerr(x0, x1) < 1e-13
 
 
 IRX = [(4, 8), (4, 64), (6, 16), (6, 64), (8, 4), (8, 32), (8, 64), (10, 16), (10, 32)]
@@ -135,9 +173,10 @@ def test_ir_yz_padded_lengths(n, sd):
     a32, b = gen32(dims, n + 7 * sd)
     with P.Handle() as h:
         x, steps = run_ir(a32, b, dims, sd, handle=h)
-        assert h.kernel_config(sd).startswith("ir thomas-pcr"), h.kernel_config(sd)
+        cfg = h.kernel_config(sd)
+        assert cfg.startswith(("ir thomas-pcr", "ir widened")), cfg
     assert relerr(x, reference(a32, b, dims, sd)) < TOL
-    assert 1 <= steps <= 5
+    assert steps_ok(cfg, steps), (cfg, steps)
 
 
 @pytest.mark.parametrize("n", [3, 5, 37, 100, 300, 500])

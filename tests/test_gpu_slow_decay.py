@@ -72,18 +72,28 @@ def test_fp32_slow_decay(dims, sd):
 
 
 @pytest.mark.parametrize("dims,sd", SHAPES)
-def test_ir_slow_decay(dims, sd):
+@pytest.mark.parametrize("algo", ["auto", "thomas-pcr"])
+def test_ir_slow_decay(dims, sd, algo):
+    """Mixed precision: the automatic choice (along y / z the FP64 solve of
+    the widened coefficients, no refinement sweep) and the refinement
+    kernels asked for by name."""
     a, b = stiff(dims, 13, 100.0)
     a32 = [v.astype(np.float32) for v in a]
-    print(f"ir stiff dims={dims} sd={sd}", flush=True)
+    print(f"ir stiff dims={dims} sd={sd} {algo}", flush=True)
     t = [torch.from_numpy(v).cuda() for v in a32]
     x = torch.from_numpy(b.copy()).cuda()
     steps = torch.zeros(1, dtype=torch.int32, device="cuda")
-    P.solve_ir(*t, x, dims, sd, max_refine=8, tol=1e-13, steps=steps)
-    torch.cuda.synchronize()
+    d = sd if len(dims) == 3 or sd == 0 else 2
+    with P.Handle() as h:
+        h.set_algo(d, algo)
+        P.solve_ir(*t, x, dims, sd, max_refine=8, tol=1e-13, steps=steps, handle=h)
+        torch.cuda.synchronize()
+        cfg = h.kernel_config(d)
     ref = O.solve(*[v.astype(np.float64) for v in a32], b.copy(), dims, sd)
     err = relerr(x.cpu().numpy(), ref)
-    print(f"  err={err:.2e} steps={int(steps.item())} kernel={P.kernel_that_ran(sd)}",
-          flush=True)
+    print(f"  err={err:.2e} steps={int(steps.item())} kernel={cfg}", flush=True)
     assert err < 1e-12
-    assert 1 <= int(steps.item()) <= 8
+    if cfg.startswith("ir widened"):
+        assert algo == "auto" and sd != 0 and int(steps.item()) == 0
+    else:
+        assert 1 <= int(steps.item()) <= 8
