@@ -11,7 +11,8 @@ summed over the three directions.
 
 Headline ("value"): FP32 bulk solve + FP64 iterative refinement (coefficients
 stored in FP32, field in FP64).  Beside it: plain FP64 and plain FP32 on the
-same driver, and the in-place drop-in per-direction FP64 solves.  Each mode
+same driver, and the in-place drop-in per-direction solves (FP64, and the
+mixed-precision entry point: FP32 coefficients, FP64 field).  Each mode
 captures --steps steps as one hipGraph (after a --warmup-step graph) and the
 timed region brackets exactly one replay of it (--steps steps) with a
 synchronisation on both sides.  Per-direction kernel
@@ -136,20 +137,32 @@ def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine,
             "one_step": result}
 
 
-def dropin_direction_ms(P, torch, coeffs, b, dims, h, reps=3):
-    """In-place per-direction solves through pentadsolver_D_gpsv_batch."""
+def dropin_direction_ms(P, torch, coeffs, b, dims, h, reps=3, ir=False):
+    """In-place per-direction solves through pentadsolver_D_gpsv_batch, or
+    (ir) the mixed-precision pentadsolver_DS_gpsv_batch_ir (FP32 coefficients,
+    FP64 field)."""
     x = b.clone()
     stream = torch.cuda.current_stream()
     out = []
     for sd in range(len(dims)):
-        nb = P.gpsv_batch_buffer_size_ext(*coeffs, x, dims, sd, h)
+        if ir:
+            nb = P.gpsv_batch_ir_buffer_size_ext(*coeffs, x, dims, sd, h)
+        else:
+            nb = P.gpsv_batch_buffer_size_ext(*coeffs, x, dims, sd, h)
         buf = torch.empty(nb, dtype=torch.uint8, device="cuda") if nb else None
-        P.gpsv_batch(*coeffs, x, dims, sd, buf, h)
+
+        def call(check=True):
+            if ir:
+                P.gpsv_batch_ir(*coeffs, x, dims, sd, buffer=buf, handle=h, check=check)
+            else:
+                P.gpsv_batch(*coeffs, x, dims, sd, buf, h, check=check)
+
+        call()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            P.gpsv_batch(*coeffs, x, dims, sd, buf, h, check=False)
+            call(check=False)
         e1.record(stream)
         torch.cuda.synchronize()
         out.append(e0.elapsed_time(e1) / reps)
@@ -619,6 +632,8 @@ def main():
             del out
     log("bench: drop-in in-place FP64 direction solves")
     dropin_ms, dropin_k = dropin_direction_ms(P, torch, c64, b, dims, h)
+    log("bench: drop-in in-place mixed-precision direction solves")
+    dropin_ir_ms, dropin_ir_k = dropin_direction_ms(P, torch, c32, b, dims, h, ir=True)
 
     check = {}
     c64_np = c32_np = b_np = None
@@ -694,6 +709,8 @@ def main():
         "ir_speedup_vs_fp64": res["fp64"]["ms_per_step"] / head["ms_per_step"],
         "dropin_fp64_inplace": {"per_direction_ms": dropin_ms, "kernels": dropin_k,
                                 "grows": 3 * n / (sum(dropin_ms) * 1e-3) / 1e9},
+        "dropin_ir_inplace": {"per_direction_ms": dropin_ir_ms, "kernels": dropin_ir_k,
+                              "grows": 3 * n / (sum(dropin_ir_ms) * 1e-3) / 1e9},
     }
     if not args.no_cpu:
         log("bench: cpu baseline (reference CPU solver)")
