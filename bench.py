@@ -333,7 +333,9 @@ def workload_c2(P, torch, W, h, check, iters=50):
 def workload_c4(P, torch, W, h, check, reps=10):
     """Batching config: 8192 systems, lengths log-uniform in [32, 8192]
     (seed 5), dominance ratio U(1.2, 3) (seed 6); one persistent launch per
-    solve vs one solve call per system (FP64)."""
+    solve vs one solve call per system (FP64).  Before each mode is timed the
+    plan's lanes per system are autotuned per length bucket on the same
+    arrays (the untuned schedule's time is reported beside)."""
     from oracle import oracle as O
     off = W.varlen_offsets(8192, 5)
     n = int(off[-1])
@@ -348,9 +350,7 @@ def workload_c4(P, torch, W, h, check, reps=10):
     res = {"systems": 8192, "rows": n}
     with P.Varlen(off, handle=h) as vl:
         res["work_items"] = vl.items()
-        for mode, cs, x0 in (("fp64", c64, bt), ("fp32", c32, bt.float()), ("ir", c32, bt)):
-            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode == "ir" else None
-            x = x0.clone()
+        def timed(cs, x, steps):
             vl.solve(*cs, x, steps=steps)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -359,10 +359,21 @@ def workload_c4(P, torch, W, h, check, reps=10):
                 vl.solve(*cs, x, steps=steps)
             e1.record(stream)
             torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / reps
+            return e0.elapsed_time(e1) / reps
+
+        for mode, cs, x0 in (("fp64", c64, bt), ("fp32", c32, bt.float()), ("ir", c32, bt)):
+            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode == "ir" else None
+            x = x0.clone()
+            vl.set_spread([0] * 10)
+            ms_default = timed(cs, x, steps)
+            x.copy_(x0)
+            vl.autotune(*cs, x, reps=3)
+            spread = vl.spread()
+            ms = timed(cs, x, steps)
             x.copy_(x0)
             vl.solve(*cs, x, steps=steps)
-            r = {"ms": ms, "grows": n / (ms * 1e-3) / 1e9}
+            r = {"ms": ms, "grows": n / (ms * 1e-3) / 1e9, "spread_per_bucket": spread,
+                 "ms_untuned": ms_default}
             if steps is not None:
                 steps.zero_()
                 x.copy_(x0)

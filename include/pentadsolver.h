@@ -393,6 +393,30 @@ pentadStatus_t pentadsolver_varlen_create(pentadsolver_handle_t handle,
 void pentadsolver_varlen_destroy(pentadsolver_varlen_t *plan);
 /* number of work items of the schedule */
 int pentadsolver_varlen_items(pentadsolver_varlen_t plan);
+/* Lanes per system by length bucket: a system of c = ceil(len / 16) chunks is
+ * in bucket b = ceil(log2 c) (0..9) and is solved by 2^min(b + spread[b], 9)
+ * lanes (each holding up to 16 rows).  spread[10], 0 by default.  The
+ * autotuners time spreads 0, 1 and 2 of every bucket present, each on the
+ * caller's arrays (x is not modified: the timed solves run on a copy), then
+ * the whole schedule with the fastest spread per bucket against the whole
+ * default one, and keep the faster in the plan.  report (may be NULL) gets
+ * one "bucket b (chunks <= c) spread s t ms" line per trial and two "whole
+ * default|tuned t ms" lines.  They allocate (the copy) and synchronise;
+ * return 0 or a negative code.  No reference counterpart. */
+pentadStatus_t pentadsolver_varlen_set_spread(pentadsolver_varlen_t plan, const int *spread);
+pentadStatus_t pentadsolver_varlen_get_spread(pentadsolver_varlen_t plan, int *spread);
+int pentadsolver_D_varlen_autotune(pentadsolver_varlen_t plan, const double *ds,
+                                   const double *dl, const double *d, const double *du,
+                                   const double *dw, const double *x, int reps,
+                                   char *report, size_t report_len);
+int pentadsolver_S_varlen_autotune(pentadsolver_varlen_t plan, const float *ds,
+                                   const float *dl, const float *d, const float *du,
+                                   const float *dw, const float *x, int reps,
+                                   char *report, size_t report_len);
+int pentadsolver_DS_varlen_autotune_ir(pentadsolver_varlen_t plan, const float *ds,
+                                       const float *dl, const float *d, const float *du,
+                                       const float *dw, const double *x, int reps,
+                                       char *report, size_t report_len);
 pentadStatus_t pentadsolver_D_gpsv_varlen_batch(pentadsolver_varlen_t plan,
                                                 const double *ds, const double *dl,
                                                 const double *d, const double *du,

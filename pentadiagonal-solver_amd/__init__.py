@@ -43,6 +43,11 @@ _SIGNATURES = {
     "pentadsolver_S_gpsv_batch_padded": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _P]),
     "pentadsolver_D_cavity_pressure_coef": (_I, [_P, _I, _I, _I] + [ctypes.c_double] * 3 + [_P] * 14),
     "pentadsolver_D_cavity_pressure_solve": (_I, [_P, _I, _I, _I] + [ctypes.c_double] * 3 + [_P] * 4),
+    "pentadsolver_varlen_set_spread": (_I, [_P, _IP]),
+    "pentadsolver_varlen_get_spread": (_I, [_P, _IP]),
+    "pentadsolver_D_varlen_autotune": (_I, [_P] * 7 + [_I, ctypes.c_char_p, _SZ]),
+    "pentadsolver_S_varlen_autotune": (_I, [_P] * 7 + [_I, ctypes.c_char_p, _SZ]),
+    "pentadsolver_DS_varlen_autotune_ir": (_I, [_P] * 7 + [_I, ctypes.c_char_p, _SZ]),
     "pentadsolver_kernel_that_ran": (ctypes.c_char_p, [_I]),
     "pentadsolver_get_last_status": (_I, [_P]),
     "pentadsolver_set_stream": (_I, [_P, _P]),
@@ -543,6 +548,41 @@ class Varlen:
 
     def items(self) -> int:
         return int(load_library().pentadsolver_varlen_items(self._plan))
+
+    def spread(self):
+        """Extra lanes per system (log2) of the 10 length buckets."""
+        out = (ctypes.c_int * 10)()
+        _check(load_library().pentadsolver_varlen_get_spread(self._plan, out))
+        return list(out)
+
+    def set_spread(self, spread) -> None:
+        arr = (ctypes.c_int * 10)(*[int(v) for v in spread])
+        _check(load_library().pentadsolver_varlen_set_spread(self._plan, arr))
+
+    def autotune(self, ds, dl, d, du, dw, x, reps=3):
+        """Time spreads 0..2 per length bucket on these arrays (x unchanged)
+        and keep the fastest; returns [(bucket, spread, ms)]."""
+        import torch
+        lib = load_library()
+        rep = ctypes.create_string_buffer(1 << 14)
+        args = [self._plan] + [_ptr(t) for t in (ds, dl, d, du, dw, x)] + [int(reps), rep,
+                                                                          len(rep)]
+        if ds.dtype == torch.float32 and x.dtype == torch.float64:
+            n = lib.pentadsolver_DS_varlen_autotune_ir(*args)
+        elif ds.dtype == x.dtype == torch.float64:
+            n = lib.pentadsolver_D_varlen_autotune(*args)
+        elif ds.dtype == x.dtype == torch.float32:
+            n = lib.pentadsolver_S_varlen_autotune(*args)
+        else:
+            raise TypeError("varlen: float64, float32, or float32 coefficients with float64 x")
+        if n < 0:
+            raise PentadError(f"varlen autotune failed ({n})")
+        out = []
+        for line in rep.value.decode().splitlines():
+            w = line.split()
+            if w[0] == "bucket":
+                out.append((int(w[1]), int(w[6]), float(w[7])))
+        return out
 
     def solve(self, ds, dl, d, du, dw, x, max_refine=5, tol=1e-13, steps=None):
         import torch

@@ -126,3 +126,71 @@ def test_varlen_graph_replay_matches_eager():
                 g.replay()
                 torch.cuda.synchronize()
                 assert np.array_equal(x.cpu().numpy(), eager), rep
+
+
+def run_mode(vl, coef, b, off, mode):
+    """(solution as float64 numpy, oracle reference, tolerance) of one solve."""
+    if mode == "fp64":
+        tc = [torch.from_numpy(a).cuda() for a in coef]
+        x = torch.from_numpy(b.copy()).cuda()
+        ref, tol = O.varlen(*coef, b, off), 1e-12
+    else:
+        c32 = [a.astype(np.float32) for a in coef]
+        tc = [torch.from_numpy(a).cuda() for a in c32]
+        w = [a.astype(np.float64) for a in c32]
+        if mode == "fp32":
+            x = torch.from_numpy(b.astype(np.float32)).cuda()
+            ref, tol = O.varlen(*w, b.astype(np.float32).astype(np.float64), off), 2e-5
+        else:
+            x = torch.from_numpy(b.copy()).cuda()
+            ref, tol = O.varlen(*w, b, off), 1e-12
+    vl.solve(*tc, x)
+    torch.cuda.synchronize()
+    return x.double().cpu().numpy(), ref, tol, tc, x
+
+
+@pytest.mark.parametrize("spread", [[1] * 10, [2] * 10, [0, 1, 2, 0, 1, 2, 0, 1, 2, 0],
+                                    [9] * 10])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir"])
+def test_varlen_spread_same_solution(spread, mode):
+    """More lanes per system than the minimum (identity-padded chunks; capped
+    at 512 lanes): the same solution, to the same bars."""
+    off = CASES["mixed"]
+    coef, b = gen(off, 11)
+    print(f"varlen spread {spread} {mode}", flush=True)
+    with P.Varlen(off) as vl:
+        vl.set_spread(spread)
+        assert vl.spread() == spread
+        got, ref, tol, _, _ = run_mode(vl, coef, b, off, mode)
+    assert relerr(got, ref) < tol
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir"])
+def test_varlen_autotune(mode):
+    """The autotuner times spreads 0..2 per length bucket on the caller's
+    arrays, leaves x unchanged, keeps the fastest per bucket (or the default
+    when the whole schedule is not faster with them), and the tuned schedule
+    solves to the same bars."""
+    off = W.varlen_offsets(600, 8, lo=3, hi=8192)
+    coef, b = gen(off, 12)
+    print(f"varlen autotune {mode}", flush=True)
+    with P.Varlen(off) as vl:
+        got, ref, tol, tc, x = run_mode(vl, coef, b, off, mode)
+        x.copy_(torch.from_numpy(b.astype(np.float32) if mode == "fp32" else b.copy()).cuda())
+        before = x.clone()
+        rep = vl.autotune(*tc, x, reps=2)
+        torch.cuda.synchronize()
+        assert torch.equal(x, before)
+        buckets = {bk for bk, _, _ in rep}
+        assert buckets and all(0 <= bk <= 9 for bk in buckets)
+        sp = vl.spread()
+        # either the fastest spread of every bucket, or (when the whole
+        # schedule with those was not faster) the default
+        if any(sp):
+            for bk in buckets:
+                times = {s: ms for b2, s, ms in rep if b2 == bk}
+                assert sp[bk] in times and times[sp[bk]] <= min(times.values()) + 1e-6
+        print(f"  spread {sp}", flush=True)
+        got2, _, _, _, _ = run_mode(vl, coef, b, off, mode)
+    assert relerr(got, ref) < tol
+    assert relerr(got2, ref) < tol
