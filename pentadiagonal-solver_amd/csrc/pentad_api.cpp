@@ -249,8 +249,10 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
       for (int k = 0; k < npref; ++k) {
         const int r = pref[k];
         // FP64 in place, 16 rows x 64 lanes (1024 rows): the LDS-staged
-        // kernel is faster (measured 0.22 against 0.32 ms for 2^23 rows)
-        if (is64 && r == 16 && ln.len == 1024 && in_place && ln.dense()) continue;
+        // kernel is faster (0.22 against 0.31 ms for 2^23 rows, 1.72 against
+        // 2.42 ms for 1024 x 256 x 256); "thomas-pcr" by name keeps it
+        if (is64 && r == 16 && ln.len == 1024 && in_place && ln.dense() && want == "auto")
+          continue;
         if (ln.len % r) continue;
         const int64_t lanes = ln.len / r;
         if (lanes > 64 || lanes < 2 || (lanes & (lanes - 1))) continue;
