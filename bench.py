@@ -263,7 +263,8 @@ def workload_c2(P, torch, W, h, check, iters=50):
     """256^3 drop-in config: 50 ADI iterations, FP64 and FP32, variable
     coefficients (seed 2) and the constant-coefficient (shared-fact) variant;
     FP64 compared with the reference's own CPU solver (oracle/_ref) and the
-    oracle after one step."""
+    oracle after one step.  The ADI driver (hipGraph replay) and, beside it,
+    the step as three in-place drop-in calls (pentadsolver_{D,S}_gpsv_batch)."""
     from oracle import oracle as O
     dims = [256] * 3
     n = 256 ** 3
@@ -282,6 +283,10 @@ def workload_c2(P, torch, W, h, check, iters=50):
             cfg = [adi.config(k) for k in range(3)]
         one[mode] = x.double().cpu().numpy()
         res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg}
+        # the same step through the drop-in entry point, in place, x -> y -> z
+        d_ms, d_k = dropin_direction_ms(P, torch, cs, x0, dims, h)
+        res[mode]["dropin"] = {"ms_per_step": sum(d_ms), "per_direction_ms": d_ms,
+                               "grows": 3 * n / (sum(d_ms) * 1e-3) / 1e9, "kernels": d_k}
     # constant-coefficient variant: one profile per direction, replicated
     g = torch.Generator(device="cuda")
     g.manual_seed(2)
