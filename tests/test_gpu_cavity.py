@@ -173,3 +173,46 @@ def test_cavity_rejects_bad_arguments():
         P.cavity_pressure_solve(1030, 4, 1030, 0.7, 1e-3, 0.25, big, big, big, xb)  # nx > 1024
     with pytest.raises(ValueError):
         P.cavity_pressure_solve(64, 64, 64, 0.7, 1e-2, 1e-2, up, vp, uv[:100], x)
+
+
+def test_cavity_graph_replay_matches_eager():
+    """The assembly, the padded solves and the fused solve captured in one
+    hipGraph (on the handle's stream): every replay equals the eager run bit
+    for bit."""
+    nx, ny, nx_pad = 100, 37, 104
+    up, vp, uv, aU, dx, dy = W.cavity_np(nx, ny, seed=5)
+    g_in = [cuda(a) for a in (up, vp, uv)]
+    dims, pads = [nx, ny], [nx_pad, ny]
+    s = torch.cuda.Stream()
+    with P.Handle(stream=s) as h:
+        outs = {k: cuda(np.zeros(nx_pad * ny)) for k in O.CAVITY_OUT}
+        xf = cuda(np.zeros(nx_pad * ny))
+        bufs = []
+        for sd, keys in ((0, XKEYS), (1, YKEYS)):
+            nb = P.gpsv_batch_padded_buffer_size(*[outs[k] for k in keys], outs["x"], dims, pads,
+                                                 sd, h)
+            bufs.append(torch.empty(nb, dtype=torch.uint8, device="cuda") if nb else None)
+
+        def step():
+            P.cavity_pressure_coef(nx, ny, nx_pad, aU, dx, dy, *g_in, outs["x"],
+                                   [outs[k] for k in XKEYS], [outs[k] for k in YKEYS], handle=h)
+            for sd, keys in ((0, XKEYS), (1, YKEYS)):
+                P.gpsv_batch_padded(*[outs[k] for k in keys], outs["x"], dims, pads, sd,
+                                    bufs[sd], h)
+            P.cavity_pressure_solve(nx, ny, nx_pad, aU, dx, dy, *g_in, xf, handle=h)
+
+        with torch.cuda.stream(s):
+            step()
+        s.synchronize()
+        eager = (outs["x"].cpu().numpy(), xf.cpu().numpy())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            step()
+        for rep in range(3):
+            for t in list(outs.values()) + [xf]:
+                t.zero_()
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(outs["x"].cpu().numpy(), eager[0]), rep
+            assert np.array_equal(xf.cpu().numpy(), eager[1]), rep
