@@ -31,11 +31,13 @@ namespace pentad {
 struct VarEx {
   unsigned char *raw; // K * N elements
   int t, p, P, N;
+  bool synced = false; // the last workgroup-wide step was all() (as LdsEx)
   template <int K, typename T>
   __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next,
-                                        int h) const {
+                                        int h) {
     T *lds = reinterpret_cast<T *>(raw);
-    __syncthreads();
+    if (!synced) __syncthreads();
+    synced = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) lds[k * N + t] = send[k];
     __syncthreads();
@@ -47,10 +49,12 @@ struct VarEx {
       from_next[k] = lds[k * N + on];
     }
   }
-  __device__ __forceinline__ bool all(bool pred) const {
-    return __syncthreads_and(pred) != 0;
+  __device__ __forceinline__ bool all(bool pred) {
+    const bool r = __syncthreads_and(pred) != 0;
+    synced = true;
+    return r;
   }
-  __device__ __forceinline__ float max_f32(float v) const {
+  __device__ __forceinline__ float max_f32(float v) {
     for (int h = 1; h < P; h <<= 1) {
       float a[1] = {v}, bp[1], bn[1];
       swapc<1>(a, bp, bn, h);

@@ -9,6 +9,12 @@
 // all(pred) is true when pred holds on every lane that takes part in the
 // exchange (wavefront or workgroup), so loops around swap stay uniform.
 // max_f32(v) is the maximum of v over the chunks of the system.
+//
+// The LDS exchanges need a barrier before their write (every lane has read
+// the previous exchange) and one after it.  all() is itself a workgroup
+// barrier that every lane reaches after its last read, so the exchange that
+// follows it skips its first barrier (the flag synced); the callers write
+// LDS only through swapc between an all() and the next exchange.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -89,12 +95,14 @@ struct LdsEx {
   unsigned char *raw; // at least K * P * S * sizeof(T) bytes
   int s, p;           // system, chunk of this thread
   int S, P;
+  bool synced = false; // the last workgroup-wide step was all()
   template <int K, typename T>
   __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next,
-                                        int h) const {
+                                        int h) {
     T *lds = reinterpret_cast<T *>(raw);
     const int PS = P * S;
-    __syncthreads();
+    if (!synced) __syncthreads();
+    synced = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) lds[k * PS + p * S + s] = send[k];
     __syncthreads();
@@ -106,8 +114,10 @@ struct LdsEx {
       from_next[k] = lds[k * PS + on];
     }
   }
-  __device__ __forceinline__ bool all(bool pred) const {
-    return __syncthreads_and(pred) != 0;
+  __device__ __forceinline__ bool all(bool pred) {
+    const bool r = __syncthreads_and(pred) != 0;
+    synced = true;
+    return r;
   }
   __device__ __forceinline__ float max_f32(float v) {
     for (int h = 1; h < P; h <<= 1) {
