@@ -64,6 +64,27 @@ struct VarEx {
   }
 };
 
+// The exchange of k_varlen: within a wavefront (WaveEx) when a system's
+// chunks fit one (P <= 64), else through LDS (VarEx).  The choice is the
+// item's, so it is uniform over the workgroup and every lane reaches the LDS
+// branch's barriers; one inlined solve serves both kinds of item (a solve
+// inlined per exchange kind spilled 148 bytes per lane in FP64, this one
+// 12).  The refinement kernel keeps one solve per kind: with this exchange it
+// spilled more (320 against 292 bytes).
+struct MixEx {
+  WaveEx wv;
+  VarEx lx;
+  bool lds;
+  template <int K, typename T>
+  __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next, int h) {
+    if (lds)
+      lx.template swapc<K>(send, from_prev, from_next, h);
+    else
+      wv.template swapc<K>(send, from_prev, from_next, h);
+  }
+  __device__ __forceinline__ bool all(bool pred) { return lds ? lx.all(pred) : wv.all(pred); }
+};
+
 // right-hand side rows of a chunk; rows past the end of the system are 0
 struct VarRhs {
   const double *p;
@@ -74,11 +95,13 @@ struct VarRhs {
 namespace {
 
 // the next item for the whole workgroup (thread 0 claims it); every
-// thread sees the same value, so all leave the loop together
+// thread sees the same value (read into a scalar register, so the item's
+// fields and the exchange choice are uniform branches), and all leave the
+// loop together
 __device__ __forceinline__ int claim_item(int *next, int *slot) {
   if (threadIdx.x == 0) *slot = atomicAdd(next, 1);
   __syncthreads();
-  const int it = *slot;
+  const int it = __builtin_amdgcn_readfirstlane(*slot);
   __syncthreads();
   return it;
 }
@@ -172,11 +195,8 @@ __global__ void __launch_bounds__(VL_BLOCK)
 #pragma unroll
       for (int i = 0; i < M; ++i) f[i] = (r0 + i < len) ? x[off + r0 + i] : T(0);
     }
-    if (P <= 64) {
-      WaveEx ex{t & 63, p, P};
-      part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
-    } else {
-      VarEx ex{smem_raw, t, p, P, VL_BLOCK};
+    {
+      MixEx ex{WaveEx{t & 63, p, P}, VarEx{smem_raw, t, p, P, VL_BLOCK}, P > 64};
       part_solve<T, M>(s, l, dd, u, w, f, p, P, ex);
     }
     if (whole) {
