@@ -366,8 +366,13 @@ def workload_c4(P, torch, W, h, check, reps=10):
             torch.cuda.synchronize()
             return e0.elapsed_time(e1) / reps
 
-        for mode, cs, x0 in (("fp64", c64, bt), ("fp32", c32, bt.float()), ("ir", c32, bt)):
-            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode == "ir" else None
+        # "ir": FP32 coefficients with an FP64 x, the plan's default (one FP64
+        # solve of the coefficients widened on load); "ir_refine": the FP32
+        # solve + FP64 refinement kernel
+        for mode, cs, x0 in (("fp64", c64, bt), ("fp32", c32, bt.float()), ("ir", c32, bt),
+                             ("ir_refine", c32, bt)):
+            vl.set_refine(mode == "ir_refine")
+            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode.startswith("ir") else None
             x = x0.clone()
             vl.set_spread([0] * 10)
             ms_default = timed(cs, x, steps)
@@ -378,7 +383,7 @@ def workload_c4(P, torch, W, h, check, reps=10):
             x.copy_(x0)
             vl.solve(*cs, x, steps=steps)
             r = {"ms": ms, "grows": n / (ms * 1e-3) / 1e9, "spread_per_bucket": spread,
-                 "ms_untuned": ms_default}
+                 "ms_untuned": ms_default, "kernel": h.kernel_config(0)}
             if steps is not None:
                 steps.zero_()
                 x.copy_(x0)

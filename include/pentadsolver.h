@@ -425,7 +425,16 @@ pentadStatus_t pentadsolver_S_gpsv_varlen_batch(pentadsolver_varlen_t plan,
                                                 const float *ds, const float *dl,
                                                 const float *d, const float *du,
                                                 const float *dw, float *x);
-/* FP32 coefficients, FP64 x; refinement as in pentadsolver_DS_gpsv_batch_ir */
+/* FP32 coefficients, FP64 x.  By default one FP64 solve of the coefficients
+ * widened on load: the widened system's solution to FP64 rounding in one
+ * pass (what the refinement converges to), no refinement sweep (steps 0,
+ * *d_refine_steps left as it is, tol and max_refine unused); the kernel
+ * config reads "varlen widened ...".  pentadsolver_varlen_set_refine(plan, 1)
+ * selects the FP32 solve + FP64 refinement of pentadsolver_DS_gpsv_batch_ir
+ * instead ("varlen ir ..."); set_refine(plan, 0) goes back.  get_refine
+ * returns the setting (-1 for a NULL plan).  No reference counterpart. */
+pentadStatus_t pentadsolver_varlen_set_refine(pentadsolver_varlen_t plan, int refine);
+int pentadsolver_varlen_get_refine(pentadsolver_varlen_t plan);
 pentadStatus_t pentadsolver_DS_gpsv_varlen_batch_ir(
     pentadsolver_varlen_t plan, const float *ds, const float *dl, const float *d,
     const float *du, const float *dw, double *x, int max_refine, double tol,

@@ -45,6 +45,8 @@ _SIGNATURES = {
     "pentadsolver_D_cavity_pressure_solve": (_I, [_P, _I, _I, _I] + [ctypes.c_double] * 3 + [_P] * 4),
     "pentadsolver_varlen_set_spread": (_I, [_P, _IP]),
     "pentadsolver_varlen_get_spread": (_I, [_P, _IP]),
+    "pentadsolver_varlen_set_refine": (_I, [_P, _I]),
+    "pentadsolver_varlen_get_refine": (_I, [_P]),
     "pentadsolver_D_varlen_autotune": (_I, [_P] * 7 + [_I, ctypes.c_char_p, _SZ]),
     "pentadsolver_S_varlen_autotune": (_I, [_P] * 7 + [_I, ctypes.c_char_p, _SZ]),
     "pentadsolver_DS_varlen_autotune_ir": (_I, [_P] * 7 + [_I, ctypes.c_char_p, _SZ]),
@@ -558,6 +560,16 @@ class Varlen:
     def set_spread(self, spread) -> None:
         arr = (ctypes.c_int * 10)(*[int(v) for v in spread])
         _check(load_library().pentadsolver_varlen_set_spread(self._plan, arr))
+
+    @property
+    def refine(self) -> bool:
+        """float32 coefficients with a float64 x: True runs the FP32 solve +
+        FP64 refinement kernel, False (the default) one FP64 solve of the
+        coefficients widened on load (no refinement sweep, steps untouched)."""
+        return load_library().pentadsolver_varlen_get_refine(self._plan) == 1
+
+    def set_refine(self, on: bool) -> None:
+        _check(load_library().pentadsolver_varlen_set_refine(self._plan, 1 if on else 0))
 
     def autotune(self, ds, dl, d, du, dw, x, reps=3):
         """Time spreads 0..2 per length bucket on these arrays (x unchanged)

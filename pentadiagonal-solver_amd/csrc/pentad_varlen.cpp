@@ -39,6 +39,7 @@ struct Varlen {
   std::vector<int64_t> len; // host, per system
   std::vector<int> order;   // host, longest first
   int spread[VL_BUCKETS] = {};
+  bool refine = false; // FP32 + refinement kernel instead of the widened FP64 solve
 };
 
 namespace {
@@ -93,6 +94,9 @@ hipError_t launch_mode(Varlen *V, int mode, const void *const c[5], void *x, con
   if (mode == 1)
     return launch_varlen<float>(a, (float *)x, V->offs, V->perm, items, n_items, V->next,
                                 V->c->stream);
+  if (!V->refine)
+    return launch_varlen_widened(a, (double *)x, V->offs, V->perm, items, n_items, V->next,
+                                 V->c->stream);
   return launch_varlen_ir(a, (double *)x, V->offs, V->perm, items, n_items, V->next, 5, 1e-13,
                           nullptr, V->c->stream);
 }
@@ -276,6 +280,17 @@ pentadStatus_t pentadsolver_varlen_set_spread(pentadsolver_varlen_t plan, const 
   return upload_items(V, build_items(*V, V->spread));
 }
 
+pentadStatus_t pentadsolver_varlen_set_refine(pentadsolver_varlen_t plan, int refine) {
+  Varlen *V = static_cast<Varlen *>(plan);
+  if (!V || refine < 0 || refine > 1) return PENTAD_STATUS_INVALID_VALUE;
+  V->refine = refine != 0;
+  return PENTAD_STATUS_SUCCESS;
+}
+
+int pentadsolver_varlen_get_refine(pentadsolver_varlen_t plan) {
+  return plan ? (static_cast<Varlen *>(plan)->refine ? 1 : 0) : -1;
+}
+
 pentadStatus_t pentadsolver_varlen_get_spread(pentadsolver_varlen_t plan, int *spread) {
   Varlen *V = static_cast<Varlen *>(plan);
   if (!V || !spread) return PENTAD_STATUS_INVALID_VALUE;
@@ -307,7 +322,8 @@ int pentadsolver_DS_varlen_autotune_ir(pentadsolver_varlen_t plan, const float *
   return autotune(static_cast<Varlen *>(plan), 2, c, x, reps, report, report_len);
 }
 
-static pentadStatus_t finish(Varlen *V, hipError_t e, const char *what) {
+static pentadStatus_t finish(Varlen *V, hipError_t e, const char *what,
+                             const char *kind = "") {
   if (e != hipSuccess) {
     std::fprintf(stderr, "[penta] %s launch FAILED: %s -- output is invalid\n", what,
                  hipGetErrorString(e));
@@ -315,7 +331,7 @@ static pentadStatus_t finish(Varlen *V, hipError_t e, const char *what) {
     return PENTAD_STATUS_EXECUTION_FAILED;
   }
   char cfg[64];
-  std::snprintf(cfg, sizeof cfg, "varlen rows=%d items=%d", VL_ROWS, V->n_items);
+  std::snprintf(cfg, sizeof cfg, "varlen%s rows=%d items=%d", kind, VL_ROWS, V->n_items);
   note_kernel(V->c, 0, "varlen", cfg);
   set_status(V->c, PENTAD_STATUS_SUCCESS);
   return PENTAD_STATUS_SUCCESS;
@@ -353,11 +369,16 @@ pentadStatus_t pentadsolver_DS_gpsv_varlen_batch_ir(pentadsolver_varlen_t plan,
   Varlen *V = static_cast<Varlen *>(plan);
   if (!V || !ds || !dl || !d || !du || !dw || !x || max_refine < 0)
     return PENTAD_STATUS_INVALID_VALUE;
+  const Diags<float> a{ds, dl, d, du, dw};
+  if (!V->refine) // no refinement sweep: *d_refine_steps is left as it is
+    return finish(V,
+                  launch_varlen_widened(a, x, V->offs, V->perm, V->items, V->n_items,
+                                        V->next, V->c->stream),
+                  "varlen FP32 widened to FP64", " widened");
   return finish(V,
-                launch_varlen_ir(Diags<float>{ds, dl, d, du, dw}, x, V->offs, V->perm,
-                                 V->items, V->n_items, V->next, max_refine, tol,
-                                 d_refine_steps, V->c->stream),
-                "varlen FP32+FP64 refinement");
+                launch_varlen_ir(a, x, V->offs, V->perm, V->items, V->n_items, V->next,
+                                 max_refine, tol, d_refine_steps, V->c->stream),
+                "varlen FP32+FP64 refinement", " ir");
 }
 
 } // extern "C"

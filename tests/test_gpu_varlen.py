@@ -3,8 +3,10 @@
 Invariant: the batch equals solving every system separately.  The oracle
 (po_varlen_d: Thomas per system over the packed offsets) is that per-system
 solve; the GPU's own per-system solves through pentadsolver_D_gpsv_batch are
-checked too.  Bars: FP64 and FP32 + FP64 refinement 1e-12 max-norm relative
-forward error; FP32 2e-5 (vs the FP64 oracle of the float32-rounded inputs).
+checked too.  Bars: FP64, and FP32 coefficients with an FP64 x (the FP64
+solve of the coefficients widened on load, and the FP32 solve + FP64
+refinement), 1e-12 max-norm relative forward error; FP32 2e-5 (vs the FP64
+oracle of the float32-rounded inputs).
 Lengths span 3 (the smallest the API accepts) to 8192 (the largest).
 """
 import numpy as np
@@ -47,8 +49,13 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir", "ir-refine"])
 def test_varlen_matches_per_system_oracle(case, mode):
+    """"ir": FP32 coefficients with an FP64 x, solved by default as one FP64
+    solve of the coefficients widened on load (no refinement sweep: the
+    steps counter stays 0); "ir-refine": the FP32 solve + FP64 refinement
+    kernel.  Both meet the FP64 bar against the oracle on the widened
+    coefficients."""
     off = CASES[case]
     print(f"varlen {case} n_sys={len(off) - 1} rows={off[-1]} {mode}", flush=True)
     coef, b = gen(off, 7)
@@ -73,14 +80,22 @@ def test_varlen_matches_per_system_oracle(case, mode):
         steps = torch.zeros(1, dtype=torch.int32, device="cuda")
     with P.Handle() as h:
         with P.Varlen(off, handle=h) as vl:
+            assert not vl.refine
+            if mode == "ir-refine":
+                vl.set_refine(True)
+                assert vl.refine
             vl.solve(*tc, x, steps=steps)
             torch.cuda.synchronize()
-            assert h.kernel_config(0).startswith("varlen")
+            cfg = h.kernel_config(0)
+    kind = {"ir": "varlen widened", "ir-refine": "varlen ir"}.get(mode, "varlen rows")
+    assert cfg.startswith(kind), cfg
     err = relerr(x.cpu().numpy(), ref)
-    print(f"  err={err:.2e}" + (f" steps={int(steps.item())}" if steps is not None else ""),
+    print(f"  err={err:.2e} {cfg}" + (f" steps={int(steps.item())}" if steps is not None else ""),
           flush=True)
     assert err < tol
-    if steps is not None:
+    if mode == "ir":
+        assert int(steps.item()) == 0
+    elif mode == "ir-refine":
         assert 1 <= int(steps.item()) <= 5
 
 
@@ -105,9 +120,12 @@ def test_varlen_rejects_bad_lengths():
             P.Varlen(offsets_of(lens))
 
 
-def test_varlen_graph_replay_matches_eager():
+@pytest.mark.parametrize("mode", ["fp64", "ir"])
+def test_varlen_graph_replay_matches_eager(mode):
     off = W.varlen_offsets(200, 4, lo=3, hi=5000)
     coef, b = gen(off, 4)
+    if mode == "ir":  # FP32 coefficients, FP64 x: the widened FP64 solve
+        coef = [a.astype(np.float32) for a in coef]
     tc = [torch.from_numpy(a).cuda() for a in coef]
     x = torch.from_numpy(b.copy()).cuda()
     with P.Varlen(off) as vl:
@@ -142,6 +160,7 @@ def run_mode(vl, coef, b, off, mode):
             x = torch.from_numpy(b.astype(np.float32)).cuda()
             ref, tol = O.varlen(*w, b.astype(np.float32).astype(np.float64), off), 2e-5
         else:
+            vl.set_refine(mode == "ir-refine")
             x = torch.from_numpy(b.copy()).cuda()
             ref, tol = O.varlen(*w, b, off), 1e-12
     vl.solve(*tc, x)
@@ -151,7 +170,7 @@ def run_mode(vl, coef, b, off, mode):
 
 @pytest.mark.parametrize("spread", [[1] * 10, [2] * 10, [0, 1, 2, 0, 1, 2, 0, 1, 2, 0],
                                     [9] * 10])
-@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir", "ir-refine"])
 def test_varlen_spread_same_solution(spread, mode):
     """More lanes per system than the minimum (identity-padded chunks; capped
     at 512 lanes): the same solution, to the same bars."""
@@ -165,7 +184,7 @@ def test_varlen_spread_same_solution(spread, mode):
     assert relerr(got, ref) < tol
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "ir", "ir-refine"])
 def test_varlen_autotune(mode):
     """The autotuner times spreads 0..2 per length bucket on the caller's
     arrays, leaves x unchanged, keeps the fastest per bucket (or the default
