@@ -88,7 +88,10 @@ __global__ void __launch_bounds__(512)
 // ROT = false: the same pipelined passes solving in place (out == b, no
 // tile; measured at 512^3: FP64 1.55 -> 1.44 ms with Q = 2, FP32 0.81 ->
 // 0.76 ms with Q = 4).
-template <typename T, int M, int Q, bool ROT>
+// NA: arrays of the next pass loaded ahead (of ds, dl, d, du, dw, b in that
+// order); the rest are loaded after the current pass's solve.  6 = all; fewer
+// where both full sets of rows do not fit the registers.
+template <typename T, int M, int Q, bool ROT, int NA = 6>
 __global__ void __launch_bounds__(256, 2)
     k_part_xq(const T *__restrict__ ds, const T *__restrict__ dl,
               const T *__restrict__ d, const T *__restrict__ du,
@@ -120,10 +123,45 @@ __global__ void __launch_bounds__(256, 2)
     RowsIO<T, M>::load(dw + offc, w);
     RowsIO<T, M>::load(b + off, f);
   };
+  // array k of pass q into dst (k: 0..4 the diagonals, 5 the right-hand side)
+  auto load_one = [&](int q, int k, T (&dst)[M]) {
+    const int64_t sys = ((blk * Q + q) << s0shift) + sl;
+    const int64_t sc = sys < n_sys ? sys : n_sys - 1;
+    const int64_t slo = map.lo(sc), shi = map.hi(sc);
+    if (k == 5) {
+      RowsIO<T, M>::load(b + slo * map.in_lo + shi * map.in_hi + (int64_t)p * M, dst);
+      return;
+    }
+    const T *src = k == 0 ? ds : k == 1 ? dl : k == 2 ? d : k == 3 ? du : dw;
+    RowsIO<T, M>::load(src + slo * map.c_lo + shi * map.c_hi + (int64_t)p * M, dst);
+  };
   T s[M], l[M], dd[M], u[M], w[M], f[M];
   load(0, s, l, dd, u, w, f);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
+    if constexpr (NA < 6) {
+      T nx[NA][M];
+#pragma unroll
+      for (int k = 0; k < NA; ++k)
+        if (q + 1 < Q) load_one(q + 1, k, nx[k]);
+      part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+      if constexpr (ROT) {
+        tile_put<T, M>(reinterpret_cast<T *>(smem_raw), f, p, Q << s0shift,
+                       (q << s0shift) + sl);
+      } else {
+        if (((blk * Q + q) << s0shift) + sl < n_sys) RowsIO<T, M>::store(out + offset(q), f);
+      }
+      if (q + 1 < Q) {
+        T(*const work[6])[M] = {&s, &l, &dd, &u, &w, &f};
+#pragma unroll
+        for (int k = NA; k < 6; ++k) load_one(q + 1, k, *work[k]);
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+#pragma unroll
+          for (int i = 0; i < M; ++i) (*work[k])[i] = nx[k][i];
+      }
+      continue;
+    }
     T ns[M], nl[M], nd[M], nu[M], nw[M], nf[M];
     if (q + 1 < Q) load(q + 1, ns, nl, nd, nu, nw, nf);
     part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
@@ -343,6 +381,31 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
         return hipGetLastError();
       };
       return Q == 2 ? go(k_part_xq<T, M, 2, true>) : go(k_part_xq<T, M, 4, true>);
+    }
+  }
+  // FP64 chunks of 10 / 12 rows: both full sets of rows do not fit, so the
+  // next pass loads two arrays ahead and the other four after the solve
+  // (384 x 256 x 640 FP64 step, 10-row chunks along z: 2.23 -> 2.06 ms; 3, 4
+  // or 5 arrays ahead measured the same)
+  if constexpr (sizeof(T) == 8 && (M == 10 || M == 12)) {
+    const int run = (256 >> lshift) * (int)sizeof(T);
+    if (run < 128) {
+      const int Q = run >= 64 ? 2 : 4;
+      const int64_t per_wg = (int64_t)(256 >> lshift) * Q;
+      const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+      const size_t smem = rotated_tile_bytes<T>(256 * Q, M, len);
+      auto go = [&](auto kernel) {
+        if (smem > 65536) {
+          const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)smem);
+          if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), smem, st, a.ds, a.dl,
+                           a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+        return hipGetLastError();
+      };
+      return Q == 2 ? go(k_part_xq<T, M, 2, true, 2>) : go(k_part_xq<T, M, 4, true, 2>);
     }
   }
   // one pass, or chunks too large to hold two sets of rows: FP32 in

@@ -72,6 +72,27 @@ def test_adi_step_matches_oracle(dims, mode):
     assert err < tol
 
 
+@pytest.mark.parametrize("dims,rows", [([640, 32, 16], 10), ([32, 320, 16], 10),
+                                       ([768, 16, 32], 12), ([16, 32, 768], 12)])
+def test_adi_fp64_partial_lookahead_chunks(dims, rows):
+    """FP64 ADI directions of 10- and 12-row chunks with 32 or 64 lanes per
+    system: the pipelined kernel that loads two arrays of the next pass ahead
+    and the other four after the solve."""
+    coef, b = gen(dims, 21)
+    print(f"adi fp64 dims={dims} rows={rows}", flush=True)
+    tc = [torch.from_numpy(a).cuda() for a in coef]
+    x = torch.from_numpy(b.copy()).cuda()
+    with P.Adi(P.ADI_FP64, dims, [tc]) as adi:
+        adi.step(x, 2)
+        torch.cuda.synchronize()
+        got = x.cpu().numpy()
+        cfg = [adi.config(k) for k in range(len(dims))]
+    err = relerr(got, oracle_steps([coef], b, dims, 2))
+    print(f"  err={err:.2e} cfg={cfg}", flush=True)
+    assert any(c.startswith(f"thomas-pcr rows={rows} ") for c in cfg), cfg  # rotating driver
+    assert err < 1e-12
+
+
 def test_adi_per_direction_coefficients_and_graph_replay():
     dims = [128, 64, 32]
     sets, b = [], None
