@@ -353,6 +353,15 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                          st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
       return hipGetLastError();
     }
+    if constexpr (sizeof(T) == 8 && (M == 10 || M == 12)) {
+      // two passes, two arrays of the next one loaded ahead (as the rotated
+      // path below)
+      const int64_t per_wg = (int64_t)(block >> lshift) * 2;
+      const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+      hipLaunchKernelGGL((k_part_xq<T, M, 2, false, 2>), dim3((unsigned)grid), dim3(block), 0,
+                         st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+      return hipGetLastError();
+    }
     const int64_t grid = (n_sys * lanes + block - 1) / block;
     hipLaunchKernelGGL((k_part_x<T, M, false, 1>), dim3((unsigned)grid), dim3(block), 0,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
