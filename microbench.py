@@ -21,9 +21,8 @@ Inputs are this library's seeded diagonally dominant recipe
 random fill (include/util/mesh.hpp:106-120), which is not diagonally
 dominant and whose per-thread seeding depends on the thread count.
 
-Usage:  python -m pentadsolver_amd.microbench [--reps 20] [--max-n 256]
-(or run this file directly).  Prints a table and, with --json, one JSON
-object per case.
+Usage:  python microbench.py [--reps 20] [--max-n 256]
+Prints a table and, with --json, one JSON object per case.
 """
 from __future__ import annotations
 
@@ -34,14 +33,15 @@ import os
 import statistics
 import sys
 
-HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "pentadiagonal-solver_amd")
 
 
 def _pkg():
     if "pentadsolver_amd" in sys.modules:
         return sys.modules["pentadsolver_amd"]
     spec = importlib.util.spec_from_file_location(
-        "pentadsolver_amd", os.path.join(HERE, "__init__.py"), submodule_search_locations=[HERE])
+        "pentadsolver_amd", os.path.join(PKG, "__init__.py"), submodule_search_locations=[PKG])
     mod = importlib.util.module_from_spec(spec)
     sys.modules["pentadsolver_amd"] = mod
     spec.loader.exec_module(mod)
@@ -114,9 +114,8 @@ def main(argv=None):
     check = None
     if not args.no_check:
         try:
-            root = os.path.dirname(HERE)
-            if root not in sys.path:
-                sys.path.insert(0, root)
+            if ROOT not in sys.path:
+                sys.path.insert(0, ROOT)
             from oracle import oracle as check  # checker only
         except ImportError:
             check = None

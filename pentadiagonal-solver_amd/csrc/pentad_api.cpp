@@ -148,7 +148,14 @@ static bool x_aligned(const Lines &ln, int elem_bytes, const void *const ptrs[6]
 Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
                 const void *const ptrs[6], bool in_place) {
   Plan pl;
-  const std::string want = requested_algo(c, ln.dir);
+  std::string want = requested_algo(c, ln.dir);
+  // x: an unrecognised name runs the automatic choice, as the reference's
+  // PENTA_XALGO does (pentad_cuda.cu:1463-1488); along y / z any name other
+  // than auto, thomas-pcr and shared-fact runs the one-lane kernel, as the
+  // reference's strided dispatch does (its "legacy")
+  if (ln.dir == 0 && want != "naive" && want != "transpose" && want != "thomas-pcr" &&
+      want != "shared-fact")
+    want = "auto";
   const bool is64 = (elem_bytes == 8);
   const TuneKey key{elem_bytes, ln.dir, ln.len};
   TuneVal tuned;
@@ -177,7 +184,8 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
   if (want == "shared-fact") {
     // opt-in only (every system is solved with system 0's coefficients, as
     // in src/cuda/singlenode/pentad_cuda.cu:962-963); a length the kernels
-    // do not cover falls back to the automatic choice below.
+    // do not cover falls back as the reference's does: x to the automatic
+    // choice (:1480-1488), y / z to the one-lane kernel (:1330-1359)
     static const int sf64[] = {8, 16, 6, 4};
     static const int sf32[] = {16, 8, 6, 4};
     const int *pref = is64 ? sf64 : sf32;
@@ -191,6 +199,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
           return pl;
         }
       }
+      want = "auto"; // x: the reference falls through to its automatic choice
     } else {
       for (int k = 0; k < 4; ++k) {
         for (int S = 64; S >= 1; S >>= 1) {
@@ -444,6 +453,17 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
   return pl;
 }
 
+const char *plan_algo_name(const Plan &pl) {
+  switch (pl.kind) {
+  case Plan::PartX:
+  case Plan::PartYZ: return "thomas-pcr";
+  case Plan::TrX: return "transpose";
+  case Plan::SfX:
+  case Plan::SfYZ: return "shared-fact";
+  default: return "naive";
+  }
+}
+
 size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes) {
   if (pl.kind == Plan::IrLane) return ir_lane_scratch_bytes(ln);
   if (pl.kind == Plan::Naive)
@@ -579,6 +599,18 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
   hipStream_t st = c->stream;
   hipError_t err = hipSuccess;
 
+  // an algorithm asked for by name that has no kernel for this shape: the
+  // automatic choice runs instead, which PENTA_DEBUG_LAUNCH=1 reports (the
+  // reference's launch_succeeded diagnostic, pentad_cuda.cu:17-37)
+  if (debug_launch()) {
+    const std::string want = requested_algo(c, ln.dir);
+    const char *got = plan_algo_name(pl);
+    if (want != "auto" && want != got)
+      std::fprintf(stderr,
+                   "[penta] %s declined along solvedim %d (length %lld) -- falling back to %s\n",
+                   want.c_str(), solvedim, (long long)ln.len, got);
+  }
+
   const bool sf = pl.kind == Plan::SfX || pl.kind == Plan::SfYZ;
   if (sf && buffer == nullptr) {
     std::fprintf(stderr,
@@ -613,11 +645,7 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
                  solvedim, hipGetErrorString(err));
     return PENTAD_STATUS_EXECUTION_FAILED;
   }
-  note_kernel(c, ln.dir,
-              pl.kind == Plan::Naive
-                  ? "naive"
-                  : (sf ? "shared-fact" : (pl.kind == Plan::TrX ? "transpose" : "thomas-pcr")),
-              plan_config(pl));
+  note_kernel(c, ln.dir, plan_algo_name(pl), plan_config(pl));
   return PENTAD_STATUS_SUCCESS;
 }
 

@@ -72,6 +72,9 @@ hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
                           const Lines &ln, void *buffer, int max_steps,
                           double tol, int *steps, hipStream_t st);
 std::string plan_config(const Plan &pl);
+// the reference's name of a plain-solve plan: "naive", "transpose",
+// "thomas-pcr" or "shared-fact" (what pentadsolver_kernel_that_ran reports)
+const char *plan_algo_name(const Plan &pl);
 // x-like partition (rows per lane, lanes per system) of the structured-
 // coefficient kernels for a line length; false when none fits
 bool stencil_plan_x(int elem_bytes, int64_t len, Plan &pl);

@@ -194,3 +194,49 @@ def test_adi_shared_fact_constant_coefficients(dims, dtype):
     err = relerr(eager.astype(np.float64), ref)
     print(f"  err={err:.2e}", flush=True)
     assert err < (1e-12 if dtype == np.float64 else 5e-5)
+
+
+# The line lengths of the 384x256x640 hyperdiffusion grid (384 = 6 rows x 64
+# lanes, 256 = 8 x 32, 640 = 10 x 64) on grids the oracle finishes in
+# seconds: each case runs the driver's kernels for two of the three lengths.
+C5_LINE_CASES = [[384, 256, 12], [640, 12, 384], [16, 640, 256]]
+
+
+@pytest.mark.parametrize("dims", C5_LINE_CASES)
+@pytest.mark.parametrize("mode", ["fp64", "ir"])
+@pytest.mark.parametrize("cond", ["well", "weak"])
+def test_adi_c5_line_lengths(dims, mode, cond):
+    """FP64 and FP32 + FP64 refinement ADI steps at the 384-, 256- and
+    640-row line lengths (the 64-lane kernels, among them the 10-row-chunk
+    refinement kernel), for the well-conditioned (ratio 2-3) and the weakly
+    dominant (c = 1.05 sum|off|) coefficient sets, against the oracle:
+    max-norm relative error <= 1e-12."""
+    ratio = (2.0, 3.0) if cond == "well" else (1.05, 1.05)
+    coef, b = gen(dims, 31 if cond == "well" else 32, ratio)
+    print(f"adi c5 lines dims={dims} mode={mode} {cond}", flush=True)
+    if mode == "fp64":
+        tc = [torch.from_numpy(a).cuda() for a in coef]
+        ref_coef = coef
+        m = P.ADI_FP64
+    else:
+        c32 = [a.astype(np.float32) for a in coef]
+        tc = [torch.from_numpy(a).cuda() for a in c32]
+        ref_coef = [a.astype(np.float64) for a in c32]
+        m = P.ADI_IR
+    x = torch.from_numpy(b.copy()).cuda()
+    with P.Adi(m, dims, [tc]) as adi:
+        adi.step(x, 1)
+        torch.cuda.synchronize()
+        got = x.cpu().numpy()
+        cfg = [adi.config(k) for k in range(3)]
+        steps = adi.refine_steps() if mode == "ir" else None
+    err = relerr(got, oracle_steps([ref_coef], b, dims, 1))
+    print(f"  err={err:.2e} cfg={cfg} steps={steps}", flush=True)
+    want = {384: "rows=6 lanes=64", 256: "rows=8 lanes=32", 640: "rows=10 lanes=64"}
+    for k, n in enumerate(dims):
+        if n in want:
+            assert cfg[k].endswith(want[n]), (n, cfg[k])
+            assert cfg[k].startswith("ir " if mode == "ir" else "thomas-pcr"), cfg[k]
+    if mode == "ir":
+        assert 1 <= steps <= 5, steps
+    assert err < 1e-12

@@ -23,11 +23,11 @@ DIMS = [48, 32, 80]
 STEPS = 10
 
 
-def oracle_run(sets, b, steps):
+def oracle_run(sets, b, steps, dims=DIMS):
     x = b.copy()
     for _ in range(steps):
         for sd in range(3):
-            x = O.solve(*sets[sd], x, DIMS, sd)
+            x = O.solve(*sets[sd], x, dims, sd)
     return x
 
 
@@ -67,3 +67,26 @@ def test_hyperdiffusion_matches_oracle(mode):
     assert np.array_equal(graph, eager)
     assert relerr(eager, ref) < 1e-12
     assert all(b2 < a2 for a2, b2 in zip(norms, norms[1:])), norms
+
+
+@pytest.mark.parametrize("dims", [[384, 256, 12], [24, 16, 640]])
+@pytest.mark.parametrize("mode", ["fp64", "ir"])
+def test_hyperdiffusion_at_full_line_lengths(dims, mode):
+    """The hyperdiffusion operators at the 384x256x640 grid's line lengths
+    (the 64- and 32-lane kernels of that workload), two steps against the
+    oracle: <= 1e-12."""
+    print(f"hyperdiffusion {dims} {mode}", flush=True)
+    sets = W.hyperdiffusion_np(dims, 7)
+    b = np.random.default_rng(8).standard_normal(int(np.prod(dims)))
+    if mode == "ir":
+        sets = [[a.astype(np.float32) for a in cs] for cs in sets]
+    tsets = [[torch.from_numpy(a).cuda() for a in cs] for cs in sets]
+    ref = oracle_run([[a.astype(np.float64) for a in cs] for cs in sets], b, 2, dims)
+    x = torch.from_numpy(b.copy()).cuda()
+    with P.Adi(P.ADI_IR if mode == "ir" else P.ADI_FP64, dims, tsets) as adi:
+        adi.step(x, 2)
+        torch.cuda.synchronize()
+        cfg = [adi.config(k) for k in range(3)]
+    err = relerr(x.cpu().numpy(), ref)
+    print(f"  kernels {cfg} err={err:.2e}", flush=True)
+    assert err < 1e-12

@@ -119,7 +119,7 @@ def test_ir_strided_auto_is_one_fp64_solve(sd):
     assert relerr(x0, x1) < 1e-13
 
 
-IRX = [(4, 8), (4, 64), (6, 16), (6, 64), (8, 4), (8, 32), (8, 64), (10, 16), (10, 32)]
+IRX = [(4, 8), (4, 64), (6, 16), (6, 64), (8, 4), (8, 32), (8, 64), (10, 16), (10, 32), (10, 64)]
 
 
 def test_ir_x_all_configs():

@@ -11,7 +11,7 @@ torch = pytest.importorskip("torch")
 
 
 def test_microbench_matrix_small():
-    mb = importlib.import_module("pentadsolver_amd.microbench")
+    mb = importlib.import_module("microbench")
     assert len(mb.matrix(256)) == 24
     rows = mb.main(["--max-n", "64", "--reps", "2", "--warmup", "1"])
     assert len(rows) == 2 * len(mb.matrix(64))

@@ -134,7 +134,8 @@ def test_naive_kernel_matches_oracle(dims, sd, dtype):
 
 # every partitioned instantiation, x: (rows, lanes) with rows*lanes = len
 X_CONFIGS = {
-    np.float64: [(4, 16), (4, 64), (6, 32), (8, 8), (8, 32), (8, 64), (10, 32), (12, 16), (16, 16), (16, 32)],
+    np.float64: [(4, 16), (4, 64), (6, 32), (8, 8), (8, 32), (8, 64), (10, 32), (12, 16), (12, 32),
+                 (12, 64), (16, 16), (16, 32)],
     np.float32: [(4, 16), (4, 64), (6, 32), (8, 8), (8, 32), (8, 64), (10, 32), (10, 64),
                  (12, 16), (16, 16), (16, 32)],
 }

@@ -5,7 +5,7 @@ import importlib
 
 
 def test_matrix_is_the_reference_matrix():
-    mb = importlib.import_module("pentadsolver_amd.microbench")
+    mb = importlib.import_module("microbench")
     cases = mb.matrix(256)
     assert len(cases) == 4 * 6
     assert sorted({c[0] for c in cases}) == [32, 64, 128, 256]
