@@ -440,6 +440,30 @@ pentadStatus_t pentadsolver_DS_gpsv_varlen_batch_ir(
     const float *du, const float *dw, double *x, int max_refine, double tol,
     int *d_refine_steps);
 
+/* Allocation-free FP64 solve: pentadsolver_D_gpsv_batch (same kernels,
+ * same results bit for bit, same errors) with the scratch taken from a fixed
+ * array in the library's own device memory instead of t_buffer, so a call
+ * allocates nothing and can be recorded under HIP graph capture.  No
+ * reference counterpart: it is for callers that cannot allocate (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Limit: a solve whose scratch need (what
+ * pentadsolver_D_gpsv_batch_buffer_size_ext reports; 0 for the on-chip
+ * kernels, 24 B per row for the one-lane kernel, the factor table for
+ * shared-fact) exceeds PENTAD_NOALLOC_SCRATCH_BYTES is refused with
+ * PENTAD_STATUS_INVALID_VALUE and a message on stderr, and nothing is
+ * launched.  Every GPU the library runs on holds one such array until the
+ * process ends (the device address is looked up at pentadsolver_create, or
+ * at the first call on a device).  All calls on one GPU share that GPU's
+ * array: calls that overlap in time on one GPU (other streams, other host
+ * threads) race and give wrong results; calls queued on one stream do not. */
+#define PENTAD_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+pentadStatus_t pentadsolver_D_gpsv_batch_noalloc(pentadsolver_handle_t handle,
+                                                 const double *ds, const double *dl,
+                                                 const double *d, const double *du,
+                                                 const double *dw, double *x,
+                                                 const int *t_dims, int t_ndim,
+                                                 int t_solvedim);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

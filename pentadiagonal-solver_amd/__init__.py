@@ -81,6 +81,7 @@ _SIGNATURES = {
     "pentadsolver_D_gpsv_varlen_batch": (_I, [_P] * 7),
     "pentadsolver_S_gpsv_varlen_batch": (_I, [_P] * 7),
     "pentadsolver_DS_gpsv_varlen_batch_ir": (_I, [_P] * 7 + [_I, ctypes.c_double, _P]),
+    "pentadsolver_D_gpsv_batch_noalloc": (_I, [_P] * 7 + [_IP, _I, _I]),
 }
 
 STATUS_NAMES = {

@@ -578,11 +578,14 @@ hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
 // the batched solve
 // ---------------------------------------------------------------------------
 // pads == nullptr: the dense layout; else the padded layout of
-// decompose_padded (extents pads[k] >= dims[k] in memory)
+// decompose_padded (extents pads[k] >= dims[k] in memory).  buffer_cap: bytes
+// the buffer holds (SIZE_MAX: the caller sized it with the matching
+// _buffer_size query); a kernel needing more is not launched.
 template <typename T>
 pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T *d,
                                   const T *du, const T *dw, T *x, const int *dims,
-                                  const int *pads, int ndim, int solvedim, void *buffer) {
+                                  const int *pads, int ndim, int solvedim, void *buffer,
+                                  size_t buffer_cap) {
   Lines ln;
   if (decompose_padded(dims, pads, ndim, solvedim, &ln) != 0 || ln.len < 3) {
     std::fprintf(stderr, "[penta] invalid shape (ndim %d, solvedim %d%s)\n",
@@ -611,14 +614,25 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
                    want.c_str(), solvedim, (long long)ln.len, got);
   }
 
+  // the scratch a plan needs is there, or nothing is launched
+  auto scratch_ok = [&](const Plan &q) {
+    const size_t need = plan_scratch_bytes(q, ln, (int)sizeof(T));
+    if (need == 0 || (buffer != nullptr && need <= buffer_cap)) return true;
+    if (buffer == nullptr)
+      std::fprintf(stderr,
+                   "[penta] %s along solvedim %d needs %zu bytes of scratch "
+                   "(pentadsolver_*_gpsv_batch_buffer_size_ext); got NULL\n",
+                   plan_algo_name(q), solvedim, need);
+    else
+      std::fprintf(stderr,
+                   "[penta] %s along solvedim %d needs %zu bytes of scratch, more than the "
+                   "%zu bytes available\n",
+                   plan_algo_name(q), solvedim, need, buffer_cap);
+    return false;
+  };
+
   const bool sf = pl.kind == Plan::SfX || pl.kind == Plan::SfYZ;
-  if (sf && buffer == nullptr) {
-    std::fprintf(stderr,
-                 "[penta] shared-fact along solvedim %d needs %zu bytes of scratch for its "
-                 "factor table (pentadsolver_*_gpsv_batch_buffer_size_ext); got NULL\n",
-                 solvedim, plan_scratch_bytes(pl, ln, (int)sizeof(T)));
-    return PENTAD_STATUS_INVALID_VALUE;
-  }
+  if (!scratch_ok(pl)) return PENTAD_STATUS_INVALID_VALUE;
   if (pl.kind != Plan::Naive) err = launch_plan<T>(pl, a, x, ln, buffer, st);
   // (a declined shared-fact launch does not fall back: the caller's buffer
   // is sized for the factor table, not for the one-lane kernel's scratch)
@@ -629,17 +643,10 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
       std::fprintf(stderr, "[penta] %s declined: %s -- falling back\n",
                    plan_config(pl).c_str(), hipGetErrorString(err));
     pl = Plan{};
+    if (!scratch_ok(pl)) return PENTAD_STATUS_INVALID_VALUE;
   }
-  if (pl.kind == Plan::Naive) {
-    if (buffer == nullptr) {
-      std::fprintf(stderr,
-                   "[penta] solvedim %d needs %zu bytes of scratch "
-                   "(pentadsolver_*_gpsv_batch_buffer_size_ext); got NULL\n",
-                   solvedim, plan_scratch_bytes(pl, ln, (int)sizeof(T)));
-      return PENTAD_STATUS_INVALID_VALUE;
-    }
+  if (pl.kind == Plan::Naive)
     err = launch_thomas_lane<T>(a, x, ln, static_cast<T *>(buffer), st);
-  }
   if (err != hipSuccess) {
     std::fprintf(stderr, "[penta] solvedim %d launch FAILED: %s -- output is invalid\n",
                  solvedim, hipGetErrorString(err));
@@ -648,6 +655,14 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
   note_kernel(c, ln.dir, plan_algo_name(pl), plan_config(pl));
   return PENTAD_STATUS_SUCCESS;
 }
+template pentadStatus_t gpsv_batch_pitched<float>(Context *, const float *, const float *,
+                                                  const float *, const float *, const float *,
+                                                  float *, const int *, const int *, int, int,
+                                                  void *, size_t);
+template pentadStatus_t gpsv_batch_pitched<double>(Context *, const double *, const double *,
+                                                   const double *, const double *,
+                                                   const double *, double *, const int *,
+                                                   const int *, int, int, void *, size_t);
 
 pentadStatus_t gpsv_batch_ir(Context *c, const float *ds, const float *dl,
                              const float *d, const float *du, const float *dw,
@@ -700,7 +715,8 @@ template <typename T>
 pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
                           const T *du, const T *dw, T *x, const int *dims,
                           int ndim, int solvedim, void *buffer) {
-  return gpsv_batch_pitched<T>(c, ds, dl, d, du, dw, x, dims, nullptr, ndim, solvedim, buffer);
+  return gpsv_batch_pitched<T>(c, ds, dl, d, du, dw, x, dims, nullptr, ndim, solvedim, buffer,
+                               SIZE_MAX);
 }
 
 template pentadStatus_t gpsv_batch<float>(Context *, const float *, const float *,
@@ -736,6 +752,10 @@ void pentadsolver_create(pentadsolver_handle_t *handle, void * /*comm*/,
   if (handle == nullptr) return;
   Context *c = new Context();
   if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  // set-up of the allocation-free solve on this device (a failure here is
+  // reported by pentadsolver_D_gpsv_batch_noalloc itself)
+  void *scratch_ptr = nullptr;
+  (void)noalloc_scratch(c->device, &scratch_ptr);
   *handle = c;
 }
 
@@ -807,7 +827,7 @@ pentadStatus_t pentadsolver_D_gpsv_batch_padded(pentadsolver_handle_t h, const d
                                                 int t_solvedim, void *t_buffer) {
   Context *c = ctx_of(h);
   const pentadStatus_t s = gpsv_batch_pitched<double>(c, ds, dl, d, du, dw, x, t_dims, t_pads,
-                                                      t_ndim, t_solvedim, t_buffer);
+                                                      t_ndim, t_solvedim, t_buffer, SIZE_MAX);
   set_status(c, s);
   return s;
 }
@@ -819,7 +839,7 @@ pentadStatus_t pentadsolver_S_gpsv_batch_padded(pentadsolver_handle_t h, const f
                                                 int t_ndim, int t_solvedim, void *t_buffer) {
   Context *c = ctx_of(h);
   const pentadStatus_t s = gpsv_batch_pitched<float>(c, ds, dl, d, du, dw, x, t_dims, t_pads,
-                                                     t_ndim, t_solvedim, t_buffer);
+                                                     t_ndim, t_solvedim, t_buffer, SIZE_MAX);
   set_status(c, s);
   return s;
 }

@@ -75,6 +75,18 @@ std::string plan_config(const Plan &pl);
 // the reference's name of a plain-solve plan: "naive", "transpose",
 // "thomas-pcr" or "shared-fact" (what pentadsolver_kernel_that_ran reports)
 const char *plan_algo_name(const Plan &pl);
+// The batched solve along one axis (pads == nullptr: the dense layout).
+// buffer holds buffer_cap bytes (SIZE_MAX: sized by the caller with the
+// matching _buffer_size query); a kernel whose scratch exceeds it is not
+// launched and the call returns PENTAD_STATUS_INVALID_VALUE.
+template <typename T>
+pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T *d,
+                                  const T *du, const T *dw, T *x, const int *dims,
+                                  const int *pads, int ndim, int solvedim, void *buffer,
+                                  size_t buffer_cap);
+// Device address of the fixed scratch array of the allocation-free solve on
+// device dev (looked up once per device; pentad_noalloc.hip).
+hipError_t noalloc_scratch(int dev, void **out);
 // x-like partition (rows per lane, lanes per system) of the structured-
 // coefficient kernels for a line length; false when none fits
 bool stencil_plan_x(int elem_bytes, int64_t len, Plan &pl);
