@@ -440,6 +440,77 @@ pentadStatus_t pentadsolver_DS_gpsv_varlen_batch_ir(
     const float *du, const float *dw, double *x, int max_refine, double tol,
     int *d_refine_steps);
 
+/* Distributed line solve, one rank's share.  Every line along t_solvedim is
+ * split across nranks ranks: rank r holds rows r*m .. r*m+m-1 of each line in
+ * its own arrays, whose shape is t_dims (t_dims[t_solvedim] = m >= 5, the
+ * same on every rank; t_pads as for pentadsolver_D_gpsv_batch_padded, NULL
+ * for the dense layout, t_ndim <= 3).  This replaces the reference's
+ * distributed pentadsolver_gpsv_batch (src/cpu/mpi/pentad_cpu.cpp:834-856,
+ * src/cuda/mpi/pentad_cuda.cu), which runs all phases with MPI inside one
+ * call; here each phase is a call and the transport between them is the
+ * caller's:
+ *  1. _dist_forward on every rank: slab-local elimination (reads the
+ *     coefficients and x, writes nothing but the record).  record receives
+ *     PENTAD_DIST_RECORD_WORDS * n_sys elements (n_sys = lines per rank),
+ *     stored [word][line]: the rank's partial 2x2 block row in its
+ *     separator (the last two rows of its slab of each line) and the
+ *     first two rows of its slab as functions of the separators around it.
+ *  2. _dist_reduced: the block tridiagonal system in the separators, one
+ *     block row per rank and line, by block cyclic reduction.  records
+ *     holds the nranks records back to back (rank r at r *
+ *     PENTAD_DIST_RECORD_WORDS * n_sys); seps receives 4 * n_sys elements
+ *     per rank, back to back: the previous rank's separator (zero on rank
+ *     0) and the rank's own, [4][n_sys].  nranks <= 64.  On several GPUs
+ *     this phase is an exchange of records between neighbours followed by
+ *     ceil(log2 nranks) rounds of the same reduction across ranks; the call
+ *     runs it for all ranks on one device.
+ *  3. _dist_backward on every rank with its 4 * n_sys separator values:
+ *     x (the right-hand side on input, untouched by phases 1 and 2) becomes
+ *     the rank's part of the solution.  The interior of each slab is solved
+ *     as an ordinary batch of lines of m - 2 rows; t_buffer holds the
+ *     scratch _dist_backward_buffer_size reports (0 for the on-chip
+ *     kernels).
+ * Asynchronous on the handle's stream; return the status.  No reference
+ * counterpart as separate calls. */
+#define PENTAD_DIST_RECORD_WORDS 23
+pentadStatus_t pentadsolver_D_dist_forward(pentadsolver_handle_t handle, const double *ds,
+                                           const double *dl, const double *d, const double *du,
+                                           const double *dw, const double *x,
+                                           const int *t_dims, const int *t_pads, int t_ndim,
+                                           int t_solvedim, int rank, int nranks,
+                                           double *record);
+pentadStatus_t pentadsolver_S_dist_forward(pentadsolver_handle_t handle, const float *ds,
+                                           const float *dl, const float *d, const float *du,
+                                           const float *dw, const float *x, const int *t_dims,
+                                           const int *t_pads, int t_ndim, int t_solvedim,
+                                           int rank, int nranks, float *record);
+pentadStatus_t pentadsolver_D_dist_reduced(pentadsolver_handle_t handle, const double *records,
+                                           int64_t n_sys, int nranks, double *seps);
+pentadStatus_t pentadsolver_S_dist_reduced(pentadsolver_handle_t handle, const float *records,
+                                           int64_t n_sys, int nranks, float *seps);
+size_t pentadsolver_D_dist_backward_buffer_size(pentadsolver_handle_t handle, const double *ds,
+                                                const double *dl, const double *d,
+                                                const double *du, const double *dw,
+                                                const double *x, const int *t_dims,
+                                                const int *t_pads, int t_ndim, int t_solvedim);
+size_t pentadsolver_S_dist_backward_buffer_size(pentadsolver_handle_t handle, const float *ds,
+                                                const float *dl, const float *d,
+                                                const float *du, const float *dw,
+                                                const float *x, const int *t_dims,
+                                                const int *t_pads, int t_ndim, int t_solvedim);
+pentadStatus_t pentadsolver_D_dist_backward(pentadsolver_handle_t handle, const double *ds,
+                                            const double *dl, const double *d, const double *du,
+                                            const double *dw, double *x, const int *t_dims,
+                                            const int *t_pads, int t_ndim, int t_solvedim,
+                                            int rank, int nranks, const double *sep,
+                                            void *t_buffer);
+pentadStatus_t pentadsolver_S_dist_backward(pentadsolver_handle_t handle, const float *ds,
+                                            const float *dl, const float *d, const float *du,
+                                            const float *dw, float *x, const int *t_dims,
+                                            const int *t_pads, int t_ndim, int t_solvedim,
+                                            int rank, int nranks, const float *sep,
+                                            void *t_buffer);
+
 /* Allocation-free FP64 solve: pentadsolver_D_gpsv_batch (same kernels,
  * same results bit for bit, same errors) with the scratch taken from a fixed
  * array in the library's own device memory instead of t_buffer, so a call

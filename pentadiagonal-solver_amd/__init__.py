@@ -19,6 +19,8 @@ import ctypes
 import os
 from typing import Optional, Sequence
 
+import numpy as np
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libpentadsolver.so")
 
@@ -82,6 +84,14 @@ _SIGNATURES = {
     "pentadsolver_S_gpsv_varlen_batch": (_I, [_P] * 7),
     "pentadsolver_DS_gpsv_varlen_batch_ir": (_I, [_P] * 7 + [_I, ctypes.c_double, _P]),
     "pentadsolver_D_gpsv_batch_noalloc": (_I, [_P] * 7 + [_IP, _I, _I]),
+    "pentadsolver_D_dist_forward": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _I, _I, _P]),
+    "pentadsolver_S_dist_forward": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _I, _I, _P]),
+    "pentadsolver_D_dist_reduced": (_I, [_P, _P, ctypes.c_int64, _I, _P]),
+    "pentadsolver_S_dist_reduced": (_I, [_P, _P, ctypes.c_int64, _I, _P]),
+    "pentadsolver_D_dist_backward_buffer_size": (_SZ, [_P] * 7 + [_IP, _IP, _I, _I]),
+    "pentadsolver_S_dist_backward_buffer_size": (_SZ, [_P] * 7 + [_IP, _IP, _I, _I]),
+    "pentadsolver_D_dist_backward": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _I, _I, _P, _P]),
+    "pentadsolver_S_dist_backward": (_I, [_P] * 7 + [_IP, _IP, _I, _I, _I, _I, _P, _P]),
 }
 
 STATUS_NAMES = {
@@ -522,6 +532,100 @@ def gpsv_stencil_batch(scale, weights, id_weight, x, dims: Sequence[int], solved
     fn = getattr(lib, f"pentadsolver_{_prefix(x.dtype)}_gpsv_stencil_batch")
     _check(fn(_hval(handle), _ptr(scale), wa, float(id_weight), _ptr(x), _dims_arg(dims),
               len(dims), int(solvedim)))
+
+
+DIST_RECORD_WORDS = 23  # PENTAD_DIST_RECORD_WORDS
+
+
+def dist_forward(ds, dl, d, du, dw, x, dims, solvedim, rank, nranks, record, pads=None,
+                 handle=None) -> None:
+    """Phase 1 of the distributed line solve on one rank's arrays (shape
+    ``dims``, optional ``pads``): writes the rank's record
+    (DIST_RECORD_WORDS * lines elements of x's dtype) into ``record``."""
+    fn = getattr(load_library(), f"pentadsolver_{_prefix(x.dtype)}_dist_forward")
+    _check(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+              _dims_arg(dims), None if pads is None else _dims_arg(pads), len(dims),
+              int(solvedim), int(rank), int(nranks), _ptr(record)))
+
+
+def dist_reduced(records, n_sys, nranks, seps, handle=None) -> None:
+    """Phase 2 for every rank on one device: ``records`` holds the nranks
+    records back to back; ``seps`` receives 4 * n_sys values per rank."""
+    fn = getattr(load_library(), f"pentadsolver_{_prefix(records.dtype)}_dist_reduced")
+    _check(fn(_hval(handle), _ptr(records), int(n_sys), int(nranks), _ptr(seps)))
+
+
+def dist_backward(ds, dl, d, du, dw, x, dims, solvedim, rank, nranks, sep, pads=None,
+                  buffer=None, handle=None) -> None:
+    """Phase 3 on one rank's arrays: x (right-hand side) becomes the rank's
+    part of the solution, from its 4 * lines separator values ``sep``."""
+    fn = getattr(load_library(), f"pentadsolver_{_prefix(x.dtype)}_dist_backward")
+    _check(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+              _dims_arg(dims), None if pads is None else _dims_arg(pads), len(dims),
+              int(solvedim), int(rank), int(nranks), _ptr(sep), _ptr(buffer)))
+
+
+def dist_backward_buffer_size(ds, dl, d, du, dw, x, dims, solvedim, pads=None,
+                              handle=None) -> int:
+    fn = getattr(load_library(), f"pentadsolver_{_prefix(x.dtype)}_dist_backward_buffer_size")
+    return int(fn(_hval(handle), _ptr(ds), _ptr(dl), _ptr(d), _ptr(du), _ptr(dw), _ptr(x),
+                  _dims_arg(dims), None if pads is None else _dims_arg(pads), len(dims),
+                  int(solvedim)))
+
+
+class DistSlabs:
+    """Every line of a field along ``solvedim`` split into ``nranks`` slabs of
+    m = dims[solvedim] / nranks rows on one GPU, slab k standing for rank
+    k's arrays (a view of the field: the rank's local shape, the field's
+    extents as pads).  ``solve`` runs the three phases of the distributed
+    line solve in place on x; the buffers (records, separators, scratch) are
+    allocated once here."""
+
+    def __init__(self, dims, solvedim, nranks, dtype, handle=None, device="cuda"):
+        import torch
+        dims = [int(v) for v in dims]
+        if dims[solvedim] % nranks:
+            raise ValueError("the line length must split into nranks equal slabs")
+        self.dims, self.sd, self.P, self.handle = dims, int(solvedim), int(nranks), handle
+        self.m = dims[solvedim] // nranks
+        self.local = list(dims)
+        self.local[solvedim] = self.m
+        self.stride = int(np.prod(dims[:solvedim])) if solvedim else 1
+        self.n_sys = int(np.prod(dims)) // dims[solvedim]
+        self.pads = None if solvedim == len(dims) - 1 else dims
+        self.records = torch.empty(nranks * DIST_RECORD_WORDS * self.n_sys, dtype=dtype,
+                                   device=device)
+        self.seps = torch.empty(nranks * 4 * self.n_sys, dtype=dtype, device=device)
+        self._buf = None
+
+    def _view(self, t, k):
+        return t.reshape(-1)[k * self.m * self.stride:]
+
+    def forward(self, ds, dl, d, du, dw, x, k):
+        rec = self.records[k * DIST_RECORD_WORDS * self.n_sys:]
+        v = [self._view(t, k) for t in (ds, dl, d, du, dw, x)]
+        dist_forward(*v, self.local, self.sd, k, self.P, rec, self.pads, self.handle)
+
+    def reduced(self):
+        dist_reduced(self.records, self.n_sys, self.P, self.seps, self.handle)
+
+    def backward(self, ds, dl, d, du, dw, x, k):
+        import torch
+        v = [self._view(t, k) for t in (ds, dl, d, du, dw, x)]
+        if self._buf is None:  # the largest need over the ranks (alignment differs)
+            nb = max(dist_backward_buffer_size(*[self._view(t, j) for t in (ds, dl, d, du, dw, x)],
+                                               self.local, self.sd, self.pads, self.handle)
+                     for j in range(self.P))
+            self._buf = torch.empty(max(nb, 1), dtype=torch.uint8, device=x.device)
+        dist_backward(*v, self.local, self.sd, k, self.P, self.seps[k * 4 * self.n_sys:],
+                      self.pads, self._buf, self.handle)
+
+    def solve(self, ds, dl, d, du, dw, x):
+        for k in range(self.P):
+            self.forward(ds, dl, d, du, dw, x, k)
+        self.reduced()
+        for k in range(self.P):
+            self.backward(ds, dl, d, du, dw, x, k)
 
 
 def kernel_that_ran(direction: int) -> str:

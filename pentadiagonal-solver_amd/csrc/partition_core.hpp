@@ -75,6 +75,46 @@ __device__ __forceinline__ bool pcr_negligible(const M2<T> &A, const M2<T> &C) {
   return fmax(a, c) <= pcr_drop_tol<T>();
 }
 
+// Block parallel cyclic reduction of a block tridiagonal system with 2x2
+// blocks, one normalised block row S_p + A S_{p-1} + C S_{p+1} = r per lane
+// (A = 0 on the first lane, C = 0 on the last), over the P lanes that Ex
+// groups; on return (r0, r1) = S_p.  Round h = 1 is peeled so that its
+// exchange is a compile-time lane shift; later rounds stop once the
+// couplings are negligible on every lane (pcr_negligible).
+template <typename T, typename Ex>
+__device__ __forceinline__ void block_pcr(M2<T> &A, M2<T> &C, T &r0, T &r1, int P, Ex &ex) {
+  auto round = [&](int h) {
+    T send[10], rp[10], rn[10];
+    send[0] = A.a00; send[1] = A.a01; send[2] = A.a10; send[3] = A.a11;
+    send[4] = C.a00; send[5] = C.a01; send[6] = C.a10; send[7] = C.a11;
+    send[8] = r0;    send[9] = r1;
+    ex.template swapc<10>(send, rp, rn, h);
+    const M2<T> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
+    const M2<T> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
+    // B' = I - A Cp - C An ; A' = -A Ap ; C' = -C Cn ; r' = r - A rp - C rn
+    const M2<T> ACp = mul(A, Cp), CAn = mul(C, An);
+    M2<T> Bn{T(1) - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
+             -ACp.a10 - CAn.a10, T(1) - ACp.a11 - CAn.a11};
+    const T q0 = r0 - (A.a00 * rp[8] + A.a01 * rp[9]) -
+                 (C.a00 * rn[8] + C.a01 * rn[9]);
+    const T q1 = r1 - (A.a10 * rp[8] + A.a11 * rp[9]) -
+                 (C.a10 * rn[8] + C.a11 * rn[9]);
+    M2<T> An2 = mul(A, Ap), Cn2 = mul(C, Cn);
+    const M2<T> Bi = inv(Bn);
+    const M2<T> nA{-An2.a00, -An2.a01, -An2.a10, -An2.a11};
+    const M2<T> nC{-Cn2.a00, -Cn2.a01, -Cn2.a10, -Cn2.a11};
+    A = mul(Bi, nA);
+    C = mul(Bi, nC);
+    r0 = Bi.a00 * q0 + Bi.a01 * q1;
+    r1 = Bi.a10 * q0 + Bi.a11 * q1;
+  };
+  if (P > 1) round(1);
+  for (int h = 2; h < P; h <<= 1) {
+    if (ex.all(pcr_negligible(A, C))) break;
+    round(h);
+  }
+}
+
 // Solve the chunk owned by this lane.  On entry s,l,d,u,w,f hold rows
 // r0..r0+M-1 of the five diagonals and the RHS; on return f holds the
 // solution of those rows.  p is the chunk index, P the chunk count.
@@ -221,37 +261,7 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
   }
 
   // ---- block parallel cyclic reduction -----------------------------------
-  // (round h = 1 peeled so that its exchange is a compile-time lane shift)
-  auto round = [&](int h) {
-    T send[10], rp[10], rn[10];
-    send[0] = A.a00; send[1] = A.a01; send[2] = A.a10; send[3] = A.a11;
-    send[4] = C.a00; send[5] = C.a01; send[6] = C.a10; send[7] = C.a11;
-    send[8] = r0;    send[9] = r1;
-    ex.template swapc<10>(send, rp, rn, h);
-    const M2<T> Ap{rp[0], rp[1], rp[2], rp[3]}, Cp{rp[4], rp[5], rp[6], rp[7]};
-    const M2<T> An{rn[0], rn[1], rn[2], rn[3]}, Cn{rn[4], rn[5], rn[6], rn[7]};
-    // B' = I - A Cp - C An ; A' = -A Ap ; C' = -C Cn ; r' = r - A rp - C rn
-    const M2<T> ACp = mul(A, Cp), CAn = mul(C, An);
-    M2<T> Bn{T(1) - ACp.a00 - CAn.a00, -ACp.a01 - CAn.a01,
-             -ACp.a10 - CAn.a10, T(1) - ACp.a11 - CAn.a11};
-    const T q0 = r0 - (A.a00 * rp[8] + A.a01 * rp[9]) -
-                 (C.a00 * rn[8] + C.a01 * rn[9]);
-    const T q1 = r1 - (A.a10 * rp[8] + A.a11 * rp[9]) -
-                 (C.a10 * rn[8] + C.a11 * rn[9]);
-    M2<T> An2 = mul(A, Ap), Cn2 = mul(C, Cn);
-    const M2<T> Bi = inv(Bn);
-    const M2<T> nA{-An2.a00, -An2.a01, -An2.a10, -An2.a11};
-    const M2<T> nC{-Cn2.a00, -Cn2.a01, -Cn2.a10, -Cn2.a11};
-    A = mul(Bi, nA);
-    C = mul(Bi, nC);
-    r0 = Bi.a00 * q0 + Bi.a01 * q1;
-    r1 = Bi.a10 * q0 + Bi.a11 * q1;
-  };
-  if (P > 1) round(1);
-  for (int h = 2; h < P; h <<= 1) {
-    if (ex.all(pcr_negligible(A, C))) break;
-    round(h);
-  }
+  block_pcr<T>(A, C, r0, r1, P, ex);
   const T t0 = r0, t1 = r1; // S_p
 
   // ---- 4. previous separator, final backward sweep -----------------------

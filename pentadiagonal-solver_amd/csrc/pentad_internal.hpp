@@ -218,6 +218,27 @@ hipError_t launch_ir_lane(const Diags<float> &a, double *x, const Lines &ln,
                           hipStream_t st);
 
 // ---------------------------------------------------------------------------
+// Distributed line solve, one rank's share (kernels_dist.hip).  A line is
+// split into P slabs of m >= 5 rows, slab k on rank k; a record holds
+// DIST_REC words per line, stored [word][line].
+// ---------------------------------------------------------------------------
+constexpr int DIST_REC = 23;
+// phase 1: this rank's record (first / last: rank 0 / rank P-1)
+template <typename T>
+hipError_t launch_dist_forward(const Diags<T> &a, const T *x, const Lines &ln, bool first,
+                               bool last, T *rec, hipStream_t st);
+// phase 2 on one device: the P records of n_sys lines, consecutive in recs;
+// seps gets per rank [4][n_sys]: the previous rank's separator and its own
+// (P <= 64)
+template <typename T>
+hipError_t launch_dist_reduced(const T *recs, int64_t n_sys, int P, T *seps, hipStream_t st);
+// phase 3, first half: separator couplings moved to the interior's end rows,
+// separator rows written (sep: this rank's [4][n_sys])
+template <typename T>
+hipError_t launch_dist_halo(const Diags<T> &a, T *x, const Lines &ln, bool first, const T *sep,
+                            hipStream_t st);
+
+// ---------------------------------------------------------------------------
 // 2-D cavity consumer: pressure-correction assembly and solve
 // (kernels_cavity.hip).  Inputs row-major with the reference's extents:
 // up = u' (ny+2) x (nx+1), vp = v' (ny+1) x (nx+2), uv (ny+2) x (nx+2);
