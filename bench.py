@@ -586,6 +586,70 @@ def workload_c5(P, torch, W, h, check, steps=100, cpu_steps=2, max_refine=5, tol
     return res
 
 
+def workload_dist(P, torch, W, h, check, size=512, nranks=8, reps=5):
+    """The distributed line solve's one-GPU share: the z lines of a size^3 FP64
+    field split into nranks slabs (slab k = rank k's size x size x size/nranks
+    arrays, a view of the field).  Per rank: phase 1 (forward elimination,
+    reads 48 B per row) and phase 3 (separator rows + the interior solve,
+    56 B per row), timed on every slab and reported as the mean per rank;
+    phase 2 (the reduced block system of all nranks ranks, which on several
+    GPUs would be spread over them) timed for the whole line set.  The whole
+    solve is checked against the oracle's z solve of the field."""
+    dims = [size, size, size]
+    ds, dl, d, du, dw, b = W.dominant_torch(dims, 3, dtype=torch.float64)
+    c = (ds, dl, d, du, dw)
+    dist = P.DistSlabs(dims, 2, nranks, torch.float64, handle=h)
+    stream = torch.cuda.current_stream()
+    x = b.clone()
+    dist.solve(*c, x)  # warm-up, and the result checked below
+    torch.cuda.synchronize()
+    out = {"field": dims, "ranks": nranks, "slab": dist.local,
+           "lines_per_rank": dist.n_sys, "rows_per_rank": dist.n_sys * dist.m}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    t_f = t_r = t_b = 0.0
+    xs = b.clone()
+    for _ in range(reps):
+        xs.copy_(b)
+        ev[0].record(stream)
+        for k in range(nranks):
+            dist.forward(*c, xs, k)
+        ev[1].record(stream)
+        dist.reduced()
+        ev[2].record(stream)
+        for k in range(nranks):
+            dist.backward(*c, xs, k)
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        t_f += ev[0].elapsed_time(ev[1])
+        t_r += ev[1].elapsed_time(ev[2])
+        t_b += ev[2].elapsed_time(ev[3])
+    t_f, t_r, t_b = t_f / reps / nranks, t_r / reps, t_b / reps / nranks
+    rows = dist.n_sys * dist.m
+    out.update({
+        "forward_ms_per_rank": t_f, "forward_GBs": 48 * rows / (t_f * 1e-3) / 1e9,
+        "backward_ms_per_rank": t_b, "backward_GBs": 56 * rows / (t_b * 1e-3) / 1e9,
+        "reduced_ms_all_ranks": t_r,
+        "rank_ms": t_f + t_b, "rank_grows": rows / ((t_f + t_b) * 1e-3) / 1e9,
+        "interior_kernel": h.kernel_config(2)})
+    # the same z lines solved whole on one GPU (the drop-in call), for scale
+    x1 = b.clone()
+    P.solve(*c, x1, dims, 2, handle=h)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        P.gpsv_batch(*c, x1, dims, 2, None, h, check=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    out["whole_line_z_ms_one_gpu"] = e0.elapsed_time(e1) / reps
+    if check:
+        from oracle import oracle as O
+        ref = O.solve(*[a.cpu().numpy() for a in c], b.cpu().numpy(), dims, 2)
+        out["max_rel_err_vs_fp64_oracle"] = max_rel(x.cpu().numpy(), ref)
+    log(f"bench: dist {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -754,6 +818,9 @@ def main():
         wl["c5_hyperdiffusion"] = workload_c5(P, torch, W, h, chk, cpu_steps=args.c5_cpu_steps,
                                               max_refine=args.max_refine,
                                               tol=args.c5_tol)
+        log("bench: distributed z solve, one GPU's share (8 slabs of 512x512x64)")
+        wl["dist_z_slabs"] = workload_dist(P, torch, W, h, chk, size=args.size)
+        torch.cuda.empty_cache()
         log("bench: 2-D cavity consumer pressure step, assembled vs fused")
         wl["cavity_pressure_step"] = workload_cavity(P, torch, W, h, chk)
         result["workloads"] = wl

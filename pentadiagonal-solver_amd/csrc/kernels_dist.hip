@@ -7,10 +7,10 @@
 // the separators known the interiors decouple, which gives three phases:
 //
 //  1. forward (per rank, k_dist_forward): the interior's first two and last
-//     two rows as affine functions of (T_{k-1}, T_k), from a top-down and a
-//     bottom-up elimination sweep over the interior (banded LU without
-//     pivoting, the recurrence of src/cpu/singlenode/pentad_orig_cpu.cpp:
-//     94-104, run from either end).  Substituted into the separator rows they
+//     two rows as affine functions of (T_{k-1}, T_k), from one top-down
+//     elimination sweep over the interior (banded LU without pivoting, the
+//     recurrence of src/cpu/singlenode/pentad_orig_cpu.cpp:94-104) that
+//     carries the first two rows along.  Substituted into the separator rows they
 //     give the rank's partial 2x2 block row; the interior's first two rows
 //     are what rank k-1 needs to complete its own.  Both go to the rank's
 //     record (DIST_REC words per line).
@@ -52,34 +52,29 @@ namespace pentad {
 //          rows 0 (wa, ub) and 1 (wb); zero on the last rank
 // DIST_REC is declared in pentad_internal.hpp.
 
-// Elimination sweep over the interior of one line, with 5 right-hand sides
-// [f, s0, s1, t0, t1].  Row i of the sweep is interior row i (down) or n-1-i
-// (up).  Returns the solution's last two swept rows, i.e. interior rows
-// n-2, n-1 (down) or 1, 0 (up), each as 5 components.
-template <typename T, bool UP>
+// One top-down elimination sweep over the interior of one line (banded LU
+// without pivoting) with 5 right-hand sides [f, s0, s1, t0, t1].  Row i
+// leaves x_i = z_i - u_i x_{i+1} - w_i x_{i+2}.  Interior rows 0 and 1 are
+// carried along as x_r = a_r + b_r x_{i+1} + g_r x_{i+2}: substituting row
+// i's relation moves them one row on (a += b z_i, b' = g - b u_i,
+// g' = -b w_i), and after the last row (whose u, w vanish) they are the
+// constants a_r.  So one pass gives the interior's last two rows (XL2, XL1)
+// and its first two (XF0, XF1), each row read once.
+template <typename T>
 __device__ __forceinline__ void dist_sweep(const Diags<T> &a, const T *__restrict__ x,
                                            int64_t b, int64_t st, int n, bool first,
-                                           T (&xa)[5], T (&xb)[5]) {
-  // state of the two previous swept rows (1 = i-1, 2 = i-2)
+                                           T (&XL2)[5], T (&XL1)[5], T (&XF0)[5],
+                                           T (&XF1)[5]) {
+  // state of the two previous rows (1 = i-1, 2 = i-2)
   T u1 = 0, w1 = 0, u2 = 0, w2 = 0;
   T z1[5] = {0, 0, 0, 0, 0}, z2[5] = {0, 0, 0, 0, 0};
-  for (int i = 0; i < n; ++i) {
-    const int r = UP ? n - 1 - i : i; // interior row
+  T b0 = 0, g0 = 0, b1 = 0, g1 = 0;
+#pragma unroll 2
+  for (int r = 0; r < n; ++r) {
     const int64_t e = b + (int64_t)r * st;
-    // couplings inside the interior, seen from the sweep's direction:
-    // back2 / back1 to the rows swept before, fwd1 / fwd2 to the rows after
-    T back2, back1, fwd1, fwd2;
-    if (!UP) {
-      back2 = r >= 2 ? a.ds[e] : T(0);
-      back1 = r >= 1 ? a.dl[e] : T(0);
-      fwd1 = r + 1 < n ? a.du[e] : T(0);
-      fwd2 = r + 2 < n ? a.dw[e] : T(0);
-    } else {
-      back2 = r + 2 < n ? a.dw[e] : T(0);
-      back1 = r + 1 < n ? a.du[e] : T(0);
-      fwd1 = r >= 1 ? a.dl[e] : T(0);
-      fwd2 = r >= 2 ? a.ds[e] : T(0);
-    }
+    // couplings inside the interior
+    const T back2 = r >= 2 ? a.ds[e] : T(0), back1 = r >= 1 ? a.dl[e] : T(0);
+    const T fwd1 = r + 1 < n ? a.du[e] : T(0), fwd2 = r + 2 < n ? a.dw[e] : T(0);
     // right-hand sides: couplings leaving the interior move across
     T rhs[5] = {x[e], T(0), T(0), T(0), T(0)};
     if (!first && r == 0) {
@@ -99,6 +94,32 @@ __device__ __forceinline__ void dist_sweep(const Diags<T> &a, const T *__restric
     T zn[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) zn[c] = (rhs[c] - back2 * z2[c] - ddl * z1[c]) * iv;
+    // rows 0 and 1 moved on past row r (uniform branches: r is the same on
+    // every lane)
+    if (r == 0) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) XF0[c] = zn[c];
+      b0 = -un;
+      g0 = -wn;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) XF0[c] += b0 * zn[c];
+      const T nb = g0 - b0 * un;
+      g0 = -b0 * wn;
+      b0 = nb;
+    }
+    if (r == 1) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) XF1[c] = zn[c];
+      b1 = -un;
+      g1 = -wn;
+    } else if (r > 1) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) XF1[c] += b1 * zn[c];
+      const T nb = g1 - b1 * un;
+      g1 = -b1 * wn;
+      b1 = nb;
+    }
     u2 = u1;
     w2 = w1;
     u1 = un;
@@ -109,28 +130,33 @@ __device__ __forceinline__ void dist_sweep(const Diags<T> &a, const T *__restric
       z1[c] = zn[c];
     }
   }
-  // the last swept row is solved; the one before it needs it (its fwd2 is 0)
+  // the last row is solved; the one before it needs it (its w is 0)
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    xb[c] = z1[c];
-    xa[c] = z2[c] - u2 * z1[c];
+    XL1[c] = z1[c];
+    XL2[c] = z2[c] - u2 * z1[c];
   }
 }
 
+// One lane per line (adjacent lines are adjacent in memory along y / z, so
+// every row load is coalesced across the wavefront).
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_dist_forward(Diags<T> a, const T *__restrict__ x, Lines ln, bool first, bool last,
                    T *__restrict__ rec) {
   const int64_t sys = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sys >= ln.n_sys) return;
   const int64_t ns = ln.n_sys, st = ln.stride;
+  if (sys >= ns) return;
   const int64_t b = ln.base(sys);
   const int n = (int)ln.len - 2; // interior rows
 
   T XL2[5], XL1[5], XF0[5], XF1[5];
-  dist_sweep<T, false>(a, x, b, st, n, first, XL2, XL1); // rows n-2, n-1
-  dist_sweep<T, true>(a, x, b, st, n, first, XF1, XF0);  // rows 1, 0
-
+  dist_sweep<T>(a, x, b, st, n, first, XL2, XL1, XF0, XF1);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    rec[c * ns + sys] = XF0[c];
+    rec[(5 + c) * ns + sys] = XF1[c];
+  }
   // separator rows n (eq a) and n+1 (eq b):
   //   sa x_{n-2} + la x_{n-1} + da t0 + ua t1 + wa y0           = f_n
   //   sb x_{n-1} + lb t0 + db t1 + ub y0 + wb y1                = f_{n+1}
@@ -141,27 +167,22 @@ __global__ void __launch_bounds__(256)
   const T wa = last ? T(0) : a.dw[ea];
   const T ub = last ? T(0) : a.du[eb];
   const T wb = last ? T(0) : a.dw[eb];
-  T out[DIST_REC];
+  T out[DIST_REC - 10];
+  out[0] = sa * XL2[1] + la * XL1[1];
+  out[1] = sa * XL2[2] + la * XL1[2];
+  out[2] = sb * XL1[1];
+  out[3] = sb * XL1[2];
+  out[4] = sa * XL2[3] + la * XL1[3] + da;
+  out[5] = sa * XL2[4] + la * XL1[4] + ua;
+  out[6] = sb * XL1[3] + lb;
+  out[7] = sb * XL1[4] + db;
+  out[8] = x[ea] - sa * XL2[0] - la * XL1[0];
+  out[9] = x[eb] - sb * XL1[0];
+  out[10] = wa;
+  out[11] = ub;
+  out[12] = wb;
 #pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    out[c] = XF0[c];
-    out[5 + c] = XF1[c];
-  }
-  out[10] = sa * XL2[1] + la * XL1[1];
-  out[11] = sa * XL2[2] + la * XL1[2];
-  out[12] = sb * XL1[1];
-  out[13] = sb * XL1[2];
-  out[14] = sa * XL2[3] + la * XL1[3] + da;
-  out[15] = sa * XL2[4] + la * XL1[4] + ua;
-  out[16] = sb * XL1[3] + lb;
-  out[17] = sb * XL1[4] + db;
-  out[18] = x[ea] - sa * XL2[0] - la * XL1[0];
-  out[19] = x[eb] - sb * XL1[0];
-  out[20] = wa;
-  out[21] = ub;
-  out[22] = wb;
-#pragma unroll
-  for (int k = 0; k < DIST_REC; ++k) rec[k * ns + sys] = out[k];
+  for (int k = 0; k < DIST_REC - 10; ++k) rec[(10 + k) * ns + sys] = out[k];
 }
 
 // Phase 2 on one device: recs holds P records of n_sys lines, record k at
