@@ -45,8 +45,19 @@ import argparse
 import importlib.util
 import json
 import os
+import subprocess
 import sys
 import time
+
+# The CPU baseline's OpenMP placement (method of the reference's CPU
+# measurements): set before any library that links an OpenMP runtime loads.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
+# CPUs this job may use, read before an OpenMP runtime binds the main thread
+try:
+    AFFINITY_CPUS = len(os.sched_getaffinity(0))
+except AttributeError:
+    AFFINITY_CPUS = None
 
 import numpy as np
 
@@ -84,18 +95,30 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine, tol):
-    """Graph-replay timing of the ADI step for one mode; returns a dict."""
+def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine, tol,
+             warmup_ms=1500.0):
+    """Graph-replay timing of the ADI step for one mode; returns a dict.
+    The warm-up graph (`warmup` steps) is replayed until at least
+    `warmup_ms` of wall time has passed (the reference's warm-up rule,
+    apps/adi_cuda.cu:403-435)."""
     mode = {"fp64": P.ADI_FP64, "fp32": P.ADI_FP32, "ir": P.ADI_IR}[mode_name]
     n = int(np.prod(dims))
     stream = torch.cuda.current_stream()
     x = b.clone()
     adi = P.Adi(mode, dims, [coeffs], handle=h, max_refine=max_refine, tol=tol)
     # warm-up graph of `warmup` steps, then the timed graph of `steps` steps
+    warm_wall = 0.0
+    warm_replays = 0
     if warmup > 0:
         adi.capture(x, warmup)
-        adi.replay(1)
-        torch.cuda.synchronize()
+        tw = time.perf_counter()
+        while True:
+            adi.replay(1)
+            torch.cuda.synchronize()
+            warm_replays += 1
+            warm_wall = (time.perf_counter() - tw) * 1e3
+            if warm_wall >= warmup_ms:
+                break
     x.copy_(b)
     adi.capture(x, steps)
     torch.cuda.synchronize()
@@ -111,7 +134,7 @@ def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine,
     ms = ev0.elapsed_time(ev1) / steps
     refine = adi.refine_steps() if mode_name == "ir" else None
     x.copy_(b)
-    dir_ms = adi.direction_ms(x, reps=3)
+    dir_ms = adi.direction_ms(x, reps=10)
     # one step from the initial field for the accuracy check
     x.copy_(b)
     adi.step(x, 1)
@@ -134,6 +157,7 @@ def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine,
             "step_GBs": bpr * n * len(dims) / (ms * 1e-3) / 1e9,
             "grows": len(dims) * n / (ms * 1e-3) / 1e9, "kernels": cfg,
             "refine_steps_timed_max": refine, "refine_steps_checked": refine_checked,
+            "warmup_wall_ms": warm_wall, "warmup_steps": warm_replays * warmup,
             "one_step": result}
 
 
@@ -178,11 +202,32 @@ def oracle_step(coeffs_np, b_np, dims):
     return x
 
 
-def cpu_baseline(coeffs_np, b_np, dims, iters=5):
-    """The reference's own OpenMP+AVX2 CPU solver (oracle/_ref, built from its
-    sources) timed on this host, FP64, method of apps/adi_cpu.cpp:49-103."""
-    from oracle import oracle as O
-    kind = "reference" if O.ref_available("avx") else "port"
+def cpu_info():
+    """lscpu model, sockets, cores and threads, plus the CPUs this process
+    may run on (the GPU box gives a job a share of the host)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets",
+                "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core",
+                "CPU(s)": "cpus", "NUMA node(s)": "numa_nodes"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    info["affinity_cpus"] = AFFINITY_CPUS
+    info["OMP_NUM_THREADS"] = os.environ.get("OMP_NUM_THREADS")
+    info["OMP_PROC_BIND"] = os.environ.get("OMP_PROC_BIND")
+    info["OMP_PLACES"] = os.environ.get("OMP_PLACES")
+    return info
+
+
+def cpu_adi_steps(O, kind, coeffs_np, b_np, dims, warm, iters):
+    """ms per direction of the reference CPU solver (or the oracle port) on
+    x/y/z in place: `warm` untimed steps, then the mean over `iters` timed
+    steps, timed per direction with a monotonic clock (apps/adi_cpu.cpp:49-103)."""
     x = b_np.copy()
 
     def solve(sd):
@@ -191,23 +236,71 @@ def cpu_baseline(coeffs_np, b_np, dims, iters=5):
         else:
             O.gpsv_batch(*coeffs_np, x, dims, sd)
 
-    for sd in range(3):  # warm-up step, faults in every page
-        solve(sd)
+    for _ in range(warm):
+        for sd in range(3):
+            solve(sd)
     t = np.zeros(3)
     for _ in range(iters):
         for sd in range(3):
             t0 = time.perf_counter()
             solve(sd)
             t[sd] += time.perf_counter() - t0
-    t /= iters
+    return t / iters * 1e3
+
+
+def cpu_baseline(coeffs_np, b_np, dims, iters=5, warm=2):
+    """The reference's own OpenMP+AVX2 CPU solver (oracle/_ref, built from its
+    sources) timed on this host, method of apps/adi_cpu.cpp:49-103: 2 warm-up
+    steps, then per-direction times averaged over the timed steps.  The value
+    is FP64 on the full grid with every thread this job may use; beside it an
+    FP32 row on the same grid and a thread sweep (1, 2, 4, ... max) of the
+    FP64 step on the 256^3 grid (a bounded sample: one thread takes about
+    16x longer).  OMP_PROC_BIND=spread, OMP_PLACES=cores."""
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available("avx") else "port"
+    info = cpu_info()
+    env_threads = os.environ.get("OMP_NUM_THREADS")
+    max_threads = int(env_threads) if env_threads else int(info.get("affinity_cpus") or 1)
+    if kind == "reference":
+        O.ref_set_threads(max_threads)
+        info["omp_proc_bind_in_force"] = O.ref_proc_bind()
     rows = 3 * int(np.prod(dims))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": float(rows / t.sum() / 1e9), "unit": "GRows/s", "cores": threads,
-            "kind": kind,
-            "sample": f"FP64 ADI step x/y/z on {dims[0]}x{dims[1]}x{dims[2]} (same seeded "
-                      f"coefficients), {iters} timed steps after 1 warm-up step, ms per "
-                      f"direction {t[0]*1e3:.1f}/{t[1]*1e3:.1f}/{t[2]*1e3:.1f}, "
-                      f"OMP_NUM_THREADS={threads}"}
+    t64 = cpu_adi_steps(O, kind, coeffs_np, b_np, dims, warm, iters)
+    out = {"value": float(rows / (t64.sum() * 1e-3) / 1e9), "unit": "GRows/s",
+           "cores": max_threads, "kind": kind,
+           "sample": f"FP64 ADI step x/y/z on {dims[0]}x{dims[1]}x{dims[2]} (same seeded "
+                     f"coefficients), {iters} timed steps after {warm} warm-up steps, ms per "
+                     f"direction {t64[0]:.1f}/{t64[1]:.1f}/{t64[2]:.1f}, {max_threads} OpenMP "
+                     f"threads, OMP_PROC_BIND={info['OMP_PROC_BIND']} "
+                     f"OMP_PLACES={info['OMP_PLACES']}",
+           "host": info,
+           "vis_GBs_fp64": [7 * 8 * rows / 3 / (t * 1e-3) / 1e9 for t in t64]}
+    if kind == "reference":
+        c32 = [a.astype(np.float32) for a in coeffs_np]
+        t32 = cpu_adi_steps(O, kind, c32, b_np.astype(np.float32), dims, warm, iters)
+        out["fp32"] = {"grows": float(rows / (t32.sum() * 1e-3) / 1e9),
+                       "ms_per_direction": t32.tolist(),
+                       "vis_GBs": [7 * 4 * rows / 3 / (t * 1e-3) / 1e9 for t in t32]}
+        del c32
+        # thread sweep on the 256^3 corner of the same field
+        q = [min(256, v) for v in dims]
+        sub = tuple(slice(0, v) for v in reversed(q))
+        shape = tuple(reversed(dims))
+        cs = [np.ascontiguousarray(a.reshape(shape)[sub]).reshape(-1) for a in coeffs_np]
+        bs = np.ascontiguousarray(b_np.reshape(shape)[sub]).reshape(-1)
+        sweep = []
+        nt = 1
+        while True:
+            O.ref_set_threads(nt)
+            t = cpu_adi_steps(O, kind, cs, bs, q, 1, 2)
+            sweep.append({"threads": nt, "ms_per_step": float(t.sum()),
+                          "grows": float(3 * int(np.prod(q)) / (t.sum() * 1e-3) / 1e9)})
+            if nt >= max_threads:
+                break
+            nt = min(2 * nt, max_threads)
+        O.ref_set_threads(max_threads)
+        out["thread_sweep_fp64"] = {"grid": q, "steps": "1 warm-up + 2 timed", "points": sweep}
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -366,13 +459,13 @@ def workload_c4(P, torch, W, h, check, reps=10):
             torch.cuda.synchronize()
             return e0.elapsed_time(e1) / reps
 
-        # "ir": FP32 coefficients with an FP64 x, the plan's default (one FP64
-        # solve of the coefficients widened on load); "ir_refine": the FP32
-        # solve + FP64 refinement kernel
+        # "ir": FP32 coefficients with an FP64 x, FP32 solve + FP64 refinement
+        # (varlen_set_refine(plan, 1)); "fp64_widened": the same coefficients
+        # solved once in FP64 after widening on load (the plan's default)
         for mode, cs, x0 in (("fp64", c64, bt), ("fp32", c32, bt.float()), ("ir", c32, bt),
-                             ("ir_refine", c32, bt)):
-            vl.set_refine(mode == "ir_refine")
-            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode.startswith("ir") else None
+                             ("fp64_widened", c32, bt)):
+            vl.set_refine(mode == "ir")
+            steps = torch.zeros(1, dtype=torch.int32, device="cuda") if mode == "ir" else None
             x = x0.clone()
             vl.set_spread([0] * 10)
             ms_default = timed(cs, x, steps)
@@ -655,6 +748,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup-ms", type=float, default=1500.0,
+                    help="replay the warm-up graph until this much wall time has passed")
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--max-refine", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-13)
@@ -698,7 +793,7 @@ def main():
     for mode, coeffs, rhs in (("ir", c32, b), ("fp64", c64, b), ("fp32", c32, b32)):
         log(f"bench: {mode} ADI step")
         res[mode] = time_adi(P, torch, mode, coeffs, rhs, dims, args.steps, args.warmup, h,
-                             args.max_refine, args.tol)
+                             args.max_refine, args.tol, args.warmup_ms)
         r = res[mode]
         log(f"bench: {mode} {r['ms_per_step']:.3f} ms/step {r['grows']:.1f} GRows/s dirs "
             f"{['%.3f' % v for v in r['dir_ms']]} kernels {r['kernels']} refine {r['refine_steps_timed_max']}")

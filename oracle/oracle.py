@@ -165,6 +165,20 @@ def ref_lib(kind: str = "avx") -> ctypes.CDLL:
     return _ref[kind]
 
 
+def ref_set_threads(n: int, kind: str = "avx") -> None:
+    """OpenMP threads of the reference solver's parallel loops."""
+    ref_lib(kind).ref_set_threads(int(n))
+
+
+def ref_threads(kind: str = "avx") -> int:
+    return int(ref_lib(kind).ref_max_threads())
+
+
+def ref_proc_bind(kind: str = "avx") -> str:
+    names = {0: "false", 1: "true", 2: "primary", 3: "close", 4: "spread"}
+    return names.get(int(ref_lib(kind).ref_proc_bind()), "unknown")
+
+
 def ref_gpsv_batch(ds, dl, d, du, dw, x, dims, solvedim, kind: str = "avx") -> None:
     """In-place solve by the reference's own CPU library.
 

@@ -45,3 +45,11 @@ extern "C" int ref_gpsv_batch_s(const float *ds, const float *dl,
                           nullptr);
   return 0;
 }
+
+// OpenMP thread count of the reference solver's parallel loops (the CPU
+// baseline's thread sweep) and the binding policy in force
+// (OMP_PROC_BIND; 0 false, 1 true, 2 primary, 3 close, 4 spread).
+#include <omp.h>
+extern "C" void ref_set_threads(int n) { omp_set_num_threads(n); }
+extern "C" int ref_max_threads(void) { return omp_get_max_threads(); }
+extern "C" int ref_proc_bind(void) { return (int)omp_get_proc_bind(); }
