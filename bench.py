@@ -152,8 +152,16 @@ def time_adi(P, torch, mode_name, coeffs, b, dims, steps, warmup, h, max_refine,
     del x
     bpr = BYTES_PER_ROW[mode_name]
     dom = int(np.argmax(dir_ms))
+    # the dominant kernel's mean launch duration inside the timed replay: the
+    # replay's time per step shared out over the directions in proportion to
+    # their event-timed durations (when every direction runs the same kernel
+    # configuration this is ms_per_step / ndim, the mean over all its launches)
+    same = all(c == cfg[dom] for c in cfg)
+    n_same = sum(1 for c in cfg if c == cfg[dom])
+    kernel_ms = (ms / n_same) if same else ms * dir_ms[dom] / sum(dir_ms)
     return {"ms_per_step": ms, "wall_ms_per_step": wall, "dir_ms": dir_ms, "dominant": dom,
-            "achieved_GBs": bpr * n / (dir_ms[dom] * 1e-3) / 1e9, "bytes_per_row": bpr,
+            "kernel_ms_timed": kernel_ms, "kernel_launches_per_step": n_same,
+            "achieved_GBs": bpr * n / (kernel_ms * 1e-3) / 1e9, "bytes_per_row": bpr,
             "step_GBs": bpr * n * len(dims) / (ms * 1e-3) / 1e9,
             "grows": len(dims) * n / (ms * 1e-3) / 1e9, "kernels": cfg,
             "refine_steps_timed_max": refine, "refine_steps_checked": refine_checked,
@@ -810,6 +818,27 @@ def main():
             gbs = 7 * out.numel() * out.element_size() / (e0.elapsed_time(e1) * 1e-3) / 1e9
             log(f"bench: probe {out.dtype}: {gbs:.0f} GB/s")
             del out
+        # the refinement solve's byte mix as 7, 3 and 2 streams (best of 10)
+        probe = {}
+        xo = torch.empty_like(b)
+        for streams in (7, 3, 2):
+            if streams == 7:
+                cf = list(c32)
+            else:
+                cf = [torch.zeros((5 if streams == 3 else 7) * n, dtype=torch.float32,
+                                  device="cuda")]
+            best = 1e30
+            for _ in range(10):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(torch.cuda.current_stream())
+                P.probe_ir_mix(streams, cf, b, xo, handle=h)
+                e1.record(torch.cuda.current_stream())
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            probe[f"{streams}_streams"] = {"ms": best, "GBs": 36 * n / (best * 1e-3) / 1e9}
+            del cf
+        del xo
+        log(f"bench: refinement byte-mix probe {probe}")
     log("bench: drop-in in-place FP64 direction solves")
     dropin_ms, dropin_k = dropin_direction_ms(P, torch, c64, b, dims, h)
     log("bench: drop-in in-place mixed-precision direction solves")
@@ -868,7 +897,12 @@ def main():
                      "algorithmic_bytes": head["bytes_per_row"] * n,
                      "kernel": f"direction {dom}: {head['kernels'][dom]}",
                      "bytes_per_row": head["bytes_per_row"],
-                     "rows_per_launch": n, "kernel_ms": head["dir_ms"][dom]},
+                     "rows_per_launch": n, "kernel_ms": head["kernel_ms_timed"],
+                     "kernel_ms_how": "mean launch duration inside the timed graph replay "
+                                      "(HIP events around the replay, time per step / "
+                                      f"{head['kernel_launches_per_step']} launches of this "
+                                      "kernel per step)",
+                     "kernel_ms_eager_events": head["dir_ms"][dom]},
         "max_rel_err_vs_fp64_oracle": check.get("ir_max_rel_err"),
         "refine_steps": {"timed_max": head["refine_steps_timed_max"],
                          "one_step_from_initial_field": head["refine_steps_checked"]},
@@ -892,6 +926,8 @@ def main():
         "dropin_ir_inplace": {"per_direction_ms": dropin_ir_ms, "kernels": dropin_ir_k,
                               "grows": 3 * n / (sum(dropin_ir_ms) * 1e-3) / 1e9},
     }
+    if args.probe:
+        result["probe_ir_byte_mix"] = probe
     if not args.no_cpu:
         log("bench: cpu baseline (reference CPU solver)")
         result["cpu_baseline"] = cpu_baseline(c64_np, b_np, dims)

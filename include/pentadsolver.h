@@ -273,6 +273,19 @@ pentadStatus_t pentadsolver_probe_stream(pentadsolver_handle_t handle,
                                          const void *const in[6], void *out,
                                          size_t n, int elem_bytes, int fmas);
 
+/* Bandwidth probe with the byte mix of the FP32 + FP64 refinement solve: n
+ * rows (n a multiple of 4), each 20 B of FP32 coefficients and 8 B of FP64 b
+ * read and 8 B of FP64 x written (x = b + the coefficients' sum), on the
+ * handle's stream.  streams = 7: coef[0..4] are five arrays of n floats;
+ * streams = 3: coef[0] holds the five interleaved per 4 rows (5n floats,
+ * 80 B per 4 rows); streams = 2: coef[0] holds coefficients and b
+ * interleaved per 4 rows (7n floats, 112 B per 4 rows) and b is not read
+ * (any valid pointer).  All pointers 16-byte aligned.  No reference
+ * counterpart (a variant of apps/roofline_probe_cuda.cu:27-49). */
+pentadStatus_t pentadsolver_probe_ir_mix(pentadsolver_handle_t handle, int streams,
+                                         const float *const coef[5], const double *b,
+                                         double *x, size_t n);
+
 /* ADI sweep driver.  One step solves along every direction in turn (x, then
  * y, then z for ndim = 3) in place on a field x in the standard layout
  * (t_dims[0] fastest), the loop of the reference's apps/adi_cuda.cu:167-176.

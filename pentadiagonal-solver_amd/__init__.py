@@ -66,6 +66,7 @@ _SIGNATURES = {
     "pentadsolver_DS_autotune_ir": (_I, [_P] * 7 + [_IP, _I, _I, _I, ctypes.c_double, _P, _SZ, _I, ctypes.c_char_p, _SZ]),
     "pentadsolver_clear_tuning": (None, [_P]),
     "pentadsolver_probe_stream": (_I, [_P, ctypes.POINTER(_P), _P, _SZ, _I, _I]),
+    "pentadsolver_probe_ir_mix": (_I, [_P, _I, ctypes.POINTER(_P), _P, _P, _SZ]),
     "pentadsolver_adi_create": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), _I, _I, _I, ctypes.c_double]),
     "pentadsolver_adi_create_stencil": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "pentadsolver_D_gpsv_stencil_batch": (_I, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_double, _P, _IP, _I, _I]),
@@ -409,6 +410,16 @@ def probe_stream(ins, out, fmas=0, handle=None):
     arr = (_P * 6)(*[_ptr(t) for t in ins])
     _check(load_library().pentadsolver_probe_stream(
         _hval(handle), arr, _ptr(out), out.numel(), out.element_size(), int(fmas)))
+
+
+def probe_ir_mix(streams, coef, b, x, handle=None):
+    """The refinement solve's byte mix (20 B FP32 in + 8 B FP64 in + 8 B FP64
+    out per row) as 7, 3 or 2 streams; coef: five float32 tensors (7) or one
+    interleaved float32 tensor (3: 5n floats, 2: 7n floats)."""
+    ptrs = [_ptr(t) for t in coef] + [0] * (5 - len(coef))
+    arr = (_P * 5)(*ptrs)
+    _check(load_library().pentadsolver_probe_ir_mix(_hval(handle), int(streams), arr, _ptr(b),
+                                                    _ptr(x), x.numel()))
 
 
 ADI_FP64, ADI_FP32, ADI_IR, ADI_SHARED_FP64, ADI_SHARED_FP32 = 0, 1, 2, 3, 4

@@ -46,4 +46,69 @@ hipError_t launch_probe(const void *const in[6], void *out, size_t n,
   return hipGetLastError();
 }
 
+// The byte mix of the FP32 + FP64 refinement solve: per row 20 B of FP32
+// coefficients and 8 B of FP64 right-hand side read, 8 B of FP64 solution
+// written, as 7 streams (five coefficient arrays, b, x), 3 streams (the
+// coefficients interleaved per 4 rows: 5 x 16 B, then b and x) or 2 streams
+// (coefficients and b interleaved per 4 rows: 5 x 16 B + 2 x 16 B, then x).
+// One thread moves 4 rows with 16-byte accesses; n must be a multiple of 4.
+__device__ __forceinline__ float sum5(float4 a, float4 b, float4 c, float4 d, float4 e, int k) {
+  const float4 *v[5] = {&a, &b, &c, &d, &e};
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) s += k == 0 ? v[j]->x : (k == 1 ? v[j]->y : (k == 2 ? v[j]->z : v[j]->w));
+  return s;
+}
+
+template <int STREAMS>
+__global__ void __launch_bounds__(256)
+    k_probe_ir_mix(const float4 *__restrict__ c0, const float4 *__restrict__ c1,
+                   const float4 *__restrict__ c2, const float4 *__restrict__ c3,
+                   const float4 *__restrict__ c4, const double2 *__restrict__ b,
+                   double2 *__restrict__ x, size_t n4) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  float4 a, bb, c, d, e;
+  double2 f0, f1;
+  if constexpr (STREAMS == 7) {
+    a = c0[i]; bb = c1[i]; c = c2[i]; d = c3[i]; e = c4[i];
+    f0 = b[2 * i]; f1 = b[2 * i + 1];
+  } else if constexpr (STREAMS == 3) {
+    const float4 *p = c0 + 5 * i;
+    a = p[0]; bb = p[1]; c = p[2]; d = p[3]; e = p[4];
+    f0 = b[2 * i]; f1 = b[2 * i + 1];
+  } else {
+    const float4 *p = c0 + 7 * i;
+    a = p[0]; bb = p[1]; c = p[2]; d = p[3]; e = p[4];
+    f0 = *reinterpret_cast<const double2 *>(p + 5);
+    f1 = *reinterpret_cast<const double2 *>(p + 6);
+  }
+  x[2 * i] = double2{f0.x + sum5(a, bb, c, d, e, 0), f0.y + sum5(a, bb, c, d, e, 1)};
+  x[2 * i + 1] = double2{f1.x + sum5(a, bb, c, d, e, 2), f1.y + sum5(a, bb, c, d, e, 3)};
+}
+
+hipError_t launch_probe_ir_mix(int streams, const float *const coef[5], const double *b,
+                               double *x, size_t n, hipStream_t st) {
+  if (n % 4) return hipErrorInvalidValue;
+  const size_t n4 = n / 4;
+  const int block = 256;
+  const size_t grid = (n4 + block - 1) / block;
+  const float4 *c[5];
+  for (int k = 0; k < 5; ++k) c[k] = reinterpret_cast<const float4 *>(coef[k]);
+  const double2 *b2 = reinterpret_cast<const double2 *>(b);
+  double2 *x2 = reinterpret_cast<double2 *>(x);
+  if (streams == 7)
+    hipLaunchKernelGGL(k_probe_ir_mix<7>, dim3((unsigned)grid), dim3(block), 0, st, c[0], c[1],
+                       c[2], c[3], c[4], b2, x2, n4);
+  else if (streams == 3)
+    hipLaunchKernelGGL(k_probe_ir_mix<3>, dim3((unsigned)grid), dim3(block), 0, st, c[0], c[0],
+                       c[0], c[0], c[0], b2, x2, n4);
+  else if (streams == 2)
+    hipLaunchKernelGGL(k_probe_ir_mix<2>, dim3((unsigned)grid), dim3(block), 0, st, c[0], c[0],
+                       c[0], c[0], c[0], b2, x2, n4);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 } // namespace pentad

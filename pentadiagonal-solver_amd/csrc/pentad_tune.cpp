@@ -24,6 +24,8 @@ namespace pentad {
 
 hipError_t launch_probe(const void *const in[6], void *out, size_t n,
                         int elem_bytes, int fmas, hipStream_t st);
+hipError_t launch_probe_ir_mix(int streams, const float *const coef[5], const double *b,
+                               double *x, size_t n, hipStream_t st);
 
 namespace {
 
@@ -292,6 +294,18 @@ pentadStatus_t pentadsolver_probe_stream(pentadsolver_handle_t h,
   for (int i = 0; i < 6; ++i)
     if (!in[i]) return PENTAD_STATUS_INVALID_VALUE;
   const hipError_t e = launch_probe(in, out, n, elem_bytes, fmas, ctx_of(h)->stream);
+  return e == hipSuccess ? PENTAD_STATUS_SUCCESS : PENTAD_STATUS_EXECUTION_FAILED;
+}
+
+pentadStatus_t pentadsolver_probe_ir_mix(pentadsolver_handle_t h, int streams,
+                                         const float *const coef[5], const double *b,
+                                         double *x, size_t n) {
+  if (!coef || !b || !x || n == 0 || n % 4 || (streams != 7 && streams != 3 && streams != 2))
+    return PENTAD_STATUS_INVALID_VALUE;
+  const int need = streams == 7 ? 5 : 1;
+  for (int i = 0; i < need; ++i)
+    if (!coef[i]) return PENTAD_STATUS_INVALID_VALUE;
+  const hipError_t e = launch_probe_ir_mix(streams, coef, b, x, n, ctx_of(h)->stream);
   return e == hipSuccess ? PENTAD_STATUS_SUCCESS : PENTAD_STATUS_EXECUTION_FAILED;
 }
 
