@@ -94,10 +94,15 @@ __device__ __forceinline__ void ir_zero_out_of_range(int k, float (&t)[M], bool 
 // SIMD; MAXR = 0 (long systems, hundreds of chunks) redoes every round.
 // The separator rows' couplings are read from the coefficients when used
 // (the last chunk's out-of-range ones are zero there).
-template <int M, int MAXR> struct IrFactors {
+// SPIKES = false drops za, zb (2 (M-2) registers): the final backward sweep
+// of every apply then redoes the forward sweep with the previous separator's
+// couplings moved into the first two rows' right-hand side, which by
+// linearity is zf + za s0 + zb s1.
+template <int M, int MAXR, bool SPIKES = true> struct IrFactors {
   static constexpr int m = M - 2;
   static constexpr int NR = MAXR > 0 ? MAXR : 1;
-  float du2[m], iv[m], za[m], zb[m];
+  static constexpr int NS = SPIKES ? m : 1;
+  float du2[m], iv[m], za[NS], zb[NS];
   int nr;    // stored rounds in use
   bool more; // rounds after the stored ones are needed (uniform over the system)
   M2<float> Bi0;
@@ -118,33 +123,34 @@ __device__ __forceinline__ M2<float> inv32(const M2<float> &b) {
 }
 
 // eliminated lower coupling of interior row i (ddl) and dw2 = w / pivot
-template <int M, int MAXR, typename Cf>
-__device__ __forceinline__ float ir_ddl(const IrFactors<M, MAXR> &F, const Cf &A, int i) {
+template <int M, int MAXR, bool SP, typename Cf>
+__device__ __forceinline__ float ir_ddl(const IrFactors<M, MAXR, SP> &F, const Cf &A, int i) {
   return (i == 1) ? A.l(1) : A.l(i) - A.s(i) * F.du2[i - 2];
 }
-template <int M, int MAXR, typename Cf>
-__device__ __forceinline__ float ir_dw2(const IrFactors<M, MAXR> &F, const Cf &A, int i) {
+template <int M, int MAXR, bool SP, typename Cf>
+__device__ __forceinline__ float ir_dw2(const IrFactors<M, MAXR, SP> &F, const Cf &A, int i) {
   return A.w(i) * F.iv[i];
 }
 
-template <int M, int MAXR, typename Cf, typename Ex>
-__device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, int p,
+template <int M, int MAXR, bool SP, typename Cf, typename Ex>
+__device__ __forceinline__ void ir_factor(IrFactors<M, MAXR, SP> &F, const Cf &A, int p,
                                           int P, Ex &ex) {
   constexpr int m = M - 2;
   const bool first = (p == 0), last = (p == P - 1);
   float dw2[m]; // kept only during the factorisation
+  float za[m], zb[m]; // the spikes, copied into F when it keeps them
   F.iv[0] = rcp32(A.d(0));
   F.du2[0] = A.u(0) * F.iv[0];
   dw2[0] = A.w(0) * F.iv[0];
-  F.za[0] = first ? 0.0f : -A.s(0) * F.iv[0];
-  F.zb[0] = first ? 0.0f : -A.l(0) * F.iv[0];
+  za[0] = first ? 0.0f : -A.s(0) * F.iv[0];
+  zb[0] = first ? 0.0f : -A.l(0) * F.iv[0];
   {
     const float l1 = A.l(1);
     F.iv[1] = rcp32(A.d(1) - l1 * F.du2[0]);
     F.du2[1] = (A.u(1) - l1 * dw2[0]) * F.iv[1];
     dw2[1] = A.w(1) * F.iv[1];
-    F.za[1] = (-l1 * F.za[0]) * F.iv[1];
-    F.zb[1] = ((first ? 0.0f : -A.s(1)) - l1 * F.zb[0]) * F.iv[1];
+    za[1] = (-l1 * za[0]) * F.iv[1];
+    zb[1] = ((first ? 0.0f : -A.s(1)) - l1 * zb[0]) * F.iv[1];
   }
 #pragma unroll
   for (int i = 2; i < m; ++i) {
@@ -153,17 +159,17 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
     F.iv[i] = rcp32(A.d(i) - si * dw2[i - 2] - dl_ * F.du2[i - 1]);
     F.du2[i] = (A.u(i) - dl_ * dw2[i - 1]) * F.iv[i];
     dw2[i] = A.w(i) * F.iv[i];
-    F.za[i] = (-si * F.za[i - 2] - dl_ * F.za[i - 1]) * F.iv[i];
-    F.zb[i] = (-si * F.zb[i - 2] - dl_ * F.zb[i - 1]) * F.iv[i];
+    za[i] = (-si * za[i - 2] - dl_ * za[i - 1]) * F.iv[i];
+    zb[i] = (-si * zb[i - 2] - dl_ * zb[i - 1]) * F.iv[i];
   }
   // backward sweep of the spike components [s0, s1, t0, t1]
   float XL1[4], XL2[4], XF0[4], XF1[4];
-  XL1[0] = F.za[m - 1];
-  XL1[1] = F.zb[m - 1];
+  XL1[0] = za[m - 1];
+  XL1[1] = zb[m - 1];
   XL1[2] = -F.du2[m - 1];
   XL1[3] = -dw2[m - 1];
-  XL2[0] = F.za[m - 2] - F.du2[m - 2] * XL1[0];
-  XL2[1] = F.zb[m - 2] - F.du2[m - 2] * XL1[1];
+  XL2[0] = za[m - 2] - F.du2[m - 2] * XL1[0];
+  XL2[1] = zb[m - 2] - F.du2[m - 2] * XL1[1];
   XL2[2] = -F.du2[m - 2] * XL1[2] - dw2[m - 2];
   XL2[3] = -F.du2[m - 2] * XL1[3];
   {
@@ -175,7 +181,7 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
     }
 #pragma unroll
     for (int i = m - 3; i >= 0; --i) {
-      const float z2[2] = {F.za[i], F.zb[i]};
+      const float z2[2] = {za[i], zb[i]};
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const float zc = (c < 2) ? z2[c] : 0.0f;
@@ -221,6 +227,13 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
   B.a11 = sb * XL1[3] + db + ub * Y0[1] + wb * Y1[1];
   C.a10 = ub * Y0[2] + wb * Y1[2];
   C.a11 = ub * Y0[3] + wb * Y1[3];
+  if constexpr (SP) {
+#pragma unroll
+    for (int i = 0; i < m; ++i) {
+      F.za[i] = za[i];
+      F.zb[i] = zb[i];
+    }
+  }
   F.Bi0 = inv32(B);
   Am = mul(F.Bi0, Am);
   C = mul(F.Bi0, C);
@@ -258,8 +271,8 @@ __device__ __forceinline__ void ir_factor(IrFactors<M, MAXR> &F, const Cf &A, in
 }
 
 // Solve A y = f with the factors; f (M rows of this chunk) is overwritten by y.
-template <int M, int MAXR, typename Cf, typename Ex>
-__device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &A,
+template <int M, int MAXR, bool SP, typename Cf, typename Ex>
+__device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR, SP> &F, const Cf &A,
                                          float (&f)[M], int p, int P, Ex &ex) {
   constexpr int m = M - 2;
   A.refresh();
@@ -339,9 +352,22 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
   }
   float xn1 = q0, xn2 = q1;
   A.refresh();
+  if constexpr (!SP) {
+    // the forward sweep again with the previous separator moved across
+    // (zero couplings on the first chunk, whose s(0), s(1), l(0) are zero)
+    zf[0] = (f[0] - A.s(0) * s0 - A.l(0) * s1) * F.iv[0];
+    zf[1] = (f[1] - A.s(1) * s1 - A.l(1) * zf[0]) * F.iv[1];
+#pragma unroll
+    for (int i = 2; i < m; ++i)
+      zf[i] = (f[i] - A.s(i) * zf[i - 2] - ir_ddl(F, A, i) * zf[i - 1]) * F.iv[i];
+  }
 #pragma unroll
   for (int i = m - 1; i >= 0; --i) {
-    const float hi = zf[i] + F.za[i] * s0 + F.zb[i] * s1;
+    float hi;
+    if constexpr (SP)
+      hi = zf[i] + F.za[i] * s0 + F.zb[i] * s1;
+    else
+      hi = zf[i];
     const float xi = hi - F.du2[i] * xn1 - ir_dw2(F, A, i) * xn2;
     f[i] = xi;
     xn2 = xn1;
@@ -354,11 +380,14 @@ __device__ __forceinline__ void ir_apply(const IrFactors<M, MAXR> &F, const Cf &
 // Full refinement on one chunk.  b: FP64 RHS rows (accessor); x: FP64
 // solution rows (output).  Returns the number of refinement steps taken
 // (uniform over the lanes that exchange with each other).
-template <int M, int MAXR, typename Cf, typename Bv, typename Ex>
-__device__ __forceinline__ int ir_solve(const Cf &A, const Bv &b, double (&x)[M],
+// x: the FP64 iterate, a register array double[M] or any X with
+// double &operator[](int) (an LDS-backed one keeps M doubles out of the
+// registers).
+template <int M, int MAXR, bool SP = true, typename Cf, typename Bv, typename X, typename Ex>
+__device__ __forceinline__ int ir_solve(const Cf &A, const Bv &b, X &x,
                                         int p, int P, Ex &ex, int max_steps,
                                         double tol) {
-  IrFactors<M, MAXR> F;
+  IrFactors<M, MAXR, SP> F;
   ir_factor<M, MAXR>(F, A, p, P, ex);
   float f[M];
 #pragma unroll

@@ -245,10 +245,10 @@ __global__ void __launch_bounds__(VL_BLOCK)
     int k;
     if (P <= 64) {
       WaveEx ex{t & 63, p, P};
-      k = ir_solve<M, 0>(A, b, xs, p, P, ex, max_steps, tol);
+      k = ir_solve<M, 0, false>(A, b, xs, p, P, ex, max_steps, tol);
     } else {
       VarEx ex{smem_raw, t, p, P, VL_BLOCK};
-      k = ir_solve<M, 0>(A, b, xs, p, P, ex, max_steps, tol);
+      k = ir_solve<M, 0, false>(A, b, xs, p, P, ex, max_steps, tol);
     }
 #pragma unroll
     for (int i = 0; i < M; ++i)
