@@ -97,8 +97,13 @@ __global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
 // coefficients in LDS runs 3 waves per SIMD but writes 32-byte pieces of
 // lines shared with three other workgroups, which the memory system merges
 // poorly (1.01 ms against 0.91 ms for this kernel at 512^3).
-template <int M, int R, int Q>
-__global__ void __launch_bounds__(256, 2)
+// BLOCK = 512 (one workgroup per CU, the same 2 waves per SIMD) serves
+// chunks of 10 rows: a 16-system tile of 640-row lines takes 82 KB of LDS,
+// more than half a CU's.  SP = false recomputes the separator spikes in every
+// apply instead of keeping them (ir_core.hpp), which the larger chunk needs
+// to stay in registers; SR is the number of stored PCR rounds (IrFactors).
+template <int M, int R, int Q, int BLOCK = 256, bool SP = true, int SR = ir_stored_rounds(R)>
+__global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     k_ir_xq(const float *__restrict__ ds, const float *__restrict__ dl,
             const float *__restrict__ d, const float *__restrict__ du,
             const float *__restrict__ dw, const double *bin, double *out,
@@ -106,7 +111,7 @@ __global__ void __launch_bounds__(256, 2)
             int *__restrict__ steps_out, XMap map) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int L = 1 << R;
-  constexpr int s0shift = 8 - R; // systems per pass (256 threads)
+  constexpr int s0shift = (BLOCK == 512 ? 9 : 8) - R; // systems per pass
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
   const int sl = (int)threadIdx.x >> R;
   const int p = (int)(threadIdx.x & (L - 1));
@@ -133,7 +138,7 @@ __global__ void __launch_bounds__(256, 2)
     const RegCoef<M> A{s, l, dd, u, w};
     const MemRhs b{bin + off, 1};
     double xs[M];
-    const int k = ir_solve<M, ir_stored_rounds(R)>(A, b, xs, p, L, ex, max_steps, tol);
+    const int k = ir_solve<M, SR, SP>(A, b, xs, p, L, ex, max_steps, tol);
     kmax = k > kmax ? k : kmax;
     tile_put<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, Q << s0shift,
                         (q << s0shift) + sl);
@@ -373,9 +378,7 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
                    int64_t n_sys, int64_t len, const XMap &map, int max_steps,
                    double tol, int *steps, hipStream_t st) {
   const int block = 256; // 512 measured slower (fewer resident workgroups)
-  // passes per workgroup so that a tile row is a whole 128-byte line (10 rows
-  // per lane would spill with the coefficients in registers: they keep the
-  // LDS-coefficient kernel below)
+  // passes per workgroup so that a tile row is a whole 128-byte line
   if constexpr (M <= 8) if (rot) {
     constexpr int run = (256 >> R) * 8;
     constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
@@ -390,6 +393,25 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
     }
     hipLaunchKernelGGL((k_ir_xq<M, R, Q>), dim3((unsigned)grid), dim3(block), smem, st, a.ds,
                        a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol, steps, map);
+    return hipGetLastError();
+  }
+  // 10 rows per lane: 512-thread workgroups (a tile of 16 systems of 640
+  // rows is 82 KB), spikes recomputed, two stored PCR rounds (255 VGPRs, no
+  // spill; three stored rounds or kept spikes spill)
+  if constexpr (M == 10) if (rot) {
+    constexpr int B = 512;
+    constexpr int run = (B >> R) * 8;
+    constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
+    const int64_t per_wg = (int64_t)(B >> R) * Q;
+    const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+    const size_t smem = rotated_tile_bytes<double>(B * Q, M, len);
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_ir_xq<M, R, Q, B, false, 2>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ir_xq<M, R, Q, B, false, 2>), dim3((unsigned)grid), dim3(B), smem, st,
+                       a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol, steps,
+                       map);
     return hipGetLastError();
   }
   const int64_t threads = n_sys << R;
