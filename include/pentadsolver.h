@@ -108,7 +108,7 @@ void pentadsolver_S_gpsv_batch(pentadsolver_handle_t handle, const float *ds,
  * pentadsolver_{D,S}_gpsv_batch; returns the status.  The buffer-size query
  * takes the same shape.  The x kernels with vector loads need pads[0] (and
  * pads[0] * pads[1]) to be multiples of 16 bytes; other pitches run the
- * one-lane kernel, which needs scratch. */
+ * workgroup-wide partitioned kernel with scalar loads (no scratch). */
 size_t pentadsolver_D_gpsv_batch_padded_buffer_size(
     pentadsolver_handle_t handle, const double *ds, const double *dl,
     const double *d, const double *du, const double *dw, const double *x,

@@ -260,9 +260,14 @@ __global__ void __launch_bounds__(MAXT)
 // rows out of LDS -- an in-tile transpose instead of a global one.  Rows past
 // a system's end (len <= L * M) are identity rows; the solution goes back
 // through the same tile.  HBM traffic is the compulsory 7 words per row.
+// WIDE: more than 64 lanes per system (lines above 64 x M rows, up to 512
+// lanes in a 512-thread workgroup): the separators are exchanged through
+// LDS across the workgroup (LdsEx, system t / L, chunk t % L), in the tile's
+// own memory, which no one reads between the last staging read and the
+// write of the solution.
 // ---------------------------------------------------------------------------
-template <typename T, int M>
-__global__ void __launch_bounds__(256)
+template <typename T, int M, bool WIDE = false>
+__global__ void __launch_bounds__(WIDE ? 512 : 256)
     k_tr_x(const T *__restrict__ ds, const T *__restrict__ dl, const T *__restrict__ d,
            const T *__restrict__ du, const T *__restrict__ dw, T *__restrict__ x,
            int64_t n_sys, int len, int lshift) {
@@ -319,8 +324,13 @@ __global__ void __launch_bounds__(256)
     const int r = p * M + i;
     f[i] = (act && r < len) ? tile[slot(own + r)] : T(0);
   }
-  WaveEx ex{(int)(threadIdx.x & 63), p, L};
-  part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+  if constexpr (WIDE) {
+    LdsEx ex{smem_raw, sl, p, S, L};
+    part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+  } else {
+    WaveEx ex{(int)(threadIdx.x & 63), p, L};
+    part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
+  }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < M; ++i) {
@@ -515,11 +525,11 @@ bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len) {
   return block <= yz_maxt_rt(elem_bytes, rows);
 }
 
-// LDS-staged x solve: rows 8 or 16, lanes a power of two <= 64, any len with
-// lanes * rows >= len > (lanes / 2) * rows (no idle chunk)
+// LDS-staged x solve: rows 8 or 16, lanes a power of two <= 512, any len
+// with lanes * rows >= len
 bool tr_x_supported(int rows, int lanes, int64_t len) {
   if (rows != 8 && rows != 16) return false;
-  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) return false;
+  if (lanes < 1 || lanes > 512 || (lanes & (lanes - 1))) return false;
   return len >= 3 && len <= (int64_t)rows * lanes;
 }
 
@@ -529,10 +539,27 @@ hipError_t launch_tr_x(const Diags<T> &a, T *x, const Lines &ln, int rows, int l
   if (!tr_x_supported(rows, lanes, ln.len) || ln.dir != 0) return hipErrorInvalidValue;
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
-  const int block = 256, S = block >> lshift;
+  const bool wide = lanes > 64;
+  const int block = wide ? 512 : 256, S = block >> lshift;
   const int64_t grid = (ln.n_sys + S - 1) / S;
   const int64_t span = (int64_t)S * ln.len;
-  const size_t smem = (size_t)(span + span / rows + 1) * sizeof(T);
+  size_t smem = (size_t)(span + span / rows + 1) * sizeof(T);
+  if (wide) { // the LDS exchange (10 values per thread) shares the tile
+    const size_t ex = (size_t)10 * block * sizeof(T);
+    if (ex > smem) smem = ex;
+    const void *fn = rows == 8 ? reinterpret_cast<const void *>(&k_tr_x<T, 8, true>)
+                               : reinterpret_cast<const void *>(&k_tr_x<T, 16, true>);
+    const hipError_t e =
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    if (rows == 8)
+      hipLaunchKernelGGL((k_tr_x<T, 8, true>), dim3((unsigned)grid), dim3(block), smem, st,
+                         a.ds, a.dl, a.d, a.du, a.dw, x, ln.n_sys, (int)ln.len, lshift);
+    else
+      hipLaunchKernelGGL((k_tr_x<T, 16, true>), dim3((unsigned)grid), dim3(block), smem, st,
+                         a.ds, a.dl, a.d, a.du, a.dw, x, ln.n_sys, (int)ln.len, lshift);
+    return hipGetLastError();
+  }
   if (rows == 8)
     hipLaunchKernelGGL((k_tr_x<T, 8>), dim3((unsigned)grid), dim3(block), smem, st, a.ds,
                        a.dl, a.d, a.du, a.dw, x, ln.n_sys, (int)ln.len, lshift);

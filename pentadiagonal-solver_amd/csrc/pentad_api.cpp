@@ -145,6 +145,20 @@ static bool x_aligned(const Lines &ln, int elem_bytes, const void *const ptrs[6]
   return ln.dense() || ((ln.lstep * elem_bytes) % 16 == 0 && (ln.gstep * elem_bytes) % 16 == 0);
 }
 
+// long x lines on the strided partitioned kernel: 16- or 12-row chunks (a
+// padded last chunk is allowed), the widest workgroup of whole systems
+bool plan_x_long(const Lines &ln, int elem_bytes, Plan &pl) {
+  for (int S = 8; S >= 1; S >>= 1)
+    for (int r : {16, 12, 8}) {
+      if (!part_yz_supported(elem_bytes, r, S, ln.len)) continue;
+      pl.kind = Plan::PartYZ;
+      pl.rows = r;
+      pl.sys_per_wg = S;
+      return true;
+    }
+  return false;
+}
+
 Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
                 const void *const ptrs[6], bool in_place) {
   Plan pl;
@@ -248,6 +262,11 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
         pl = tuned;
         return pl;
       }
+      if (have_tuned && tuned.kind == Plan::PartYZ &&
+          part_yz_supported(elem_bytes, tuned.rows, tuned.sys_per_wg, ln.len)) {
+        pl = tuned;
+        return pl;
+      }
       // preferred rows per lane: 8 (FP64) / 16 (FP32), then the others
       // (FP32 lines of 10 * 2^k rows on the LDS-staged kernel instead: 640
       // rows 1.07 ms against 0.36 ms with 10 rows per lane)
@@ -272,7 +291,15 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
         return pl;
       }
     }
+    // lines longer than one wavefront's chunks cover (above 64 x 16 rows):
+    // "thomas-pcr" by name, and the automatic choice where the LDS-staged
+    // kernel cannot stage (padded layouts), take the workgroup-wide
+    // partitioned kernel of the strided directions, whose chunks are
+    // contiguous rows here (stride 1), separators exchanged through LDS; up
+    // to 8192 rows (FP64) in one workgroup, no scratch
+    if (want == "thomas-pcr" && ln.len > 16 * 64 && plan_x_long(ln, elem_bytes, pl)) return pl;
     if ((want == "auto" || want == "thomas-pcr") && plan_tr(pl)) return pl;
+    if ((want == "auto" || want == "thomas-pcr") && plan_x_long(ln, elem_bytes, pl)) return pl;
     pl.kind = Plan::Naive;
     return pl;
   }

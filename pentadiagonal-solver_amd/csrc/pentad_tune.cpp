@@ -90,6 +90,17 @@ std::vector<Plan> candidates(const Lines &ln, int elem_bytes, bool ir,
           out.push_back(p);
         }
   }
+  if (!ir && ln.dir == 0 && ln.len > 16 * 64) { // long lines: the workgroup-wide kernel
+    for (int r : {8, 12, 16})
+      for (int S = 1; S <= 8; S <<= 1)
+        if (part_yz_supported(elem_bytes, r, S, ln.len)) {
+          Plan p;
+          p.kind = Plan::PartYZ;
+          p.rows = r;
+          p.sys_per_wg = S;
+          out.push_back(p);
+        }
+  }
   if (!ir && ln.dir == 0) { // LDS-staged solve ("transpose"), any length
     for (int r : {8, 16}) {
       int lanes = 1;

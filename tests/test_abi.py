@@ -77,9 +77,14 @@ def test_host_only_calls_without_gpu():
     # odd length: the partitioned kernel pads its last chunk, no scratch
     dims2 = (ctypes.c_int * 2)(8, 37)
     assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims2, 2, 1) == 0
-    # a length past every on-chip kernel falls back to the scratch kernel
+    # long x lines: the workgroup-wide partitioned kernel, no scratch
     dims3 = (ctypes.c_int * 2)(3000, 8)
-    assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims3, 2, 0) == 3 * 8 * 3000 * 4
+    assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims3, 2, 0) == 0
+    dims3 = (ctypes.c_int * 2)(8192, 8)
+    assert lib.pentadsolver_D_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims3, 2, 0) == 0
+    # a length past every on-chip kernel falls back to the scratch kernel
+    dims4 = (ctypes.c_int * 2)(13000, 8)
+    assert lib.pentadsolver_S_gpsv_batch_buffer_size_ext(h, *([None] * 6), dims4, 2, 0) == 3 * 8 * 13000 * 4
     assert lib.pentadsolver_kernel_that_ran(5) == b"none"
     lib.pentadsolver_destroy(ctypes.byref(h))
     assert not h.value

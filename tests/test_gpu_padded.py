@@ -49,16 +49,18 @@ def relerr(a, ref):
     return float(np.max(np.abs(a.astype(np.float64) - ref)) / np.max(np.abs(ref)))
 
 
-# (dims, pads, dtype, expected kernel family along x)
+# (dims, pads, dtype, expected kernel along x): "lanes" = the wavefront
+# kernels with 16-byte vector loads, "sys_per_wg" = the workgroup-wide
+# partitioned kernel with scalar loads (any pitch, any length, no scratch)
 CASES = [
-    ([64, 48, 40], [72, 50, 40], np.float64, "thomas-pcr"),  # pitches multiples of 16 B
-    ([64, 48, 40], [67, 48, 40], np.float64, "naive"),       # 536-byte rows: one lane
-    ([128, 30, 20], [132, 33, 20], np.float32, "thomas-pcr"),
-    ([128, 30, 20], [130, 30, 20], np.float32, "naive"),
-    ([256, 64], [260, 64], np.float64, "thomas-pcr"),
-    ([100, 37], [104, 37], np.float64, "naive"),             # no whole partition of 100
-    ([30, 7, 3], [32, 7, 3], np.float64, "naive"),           # the reference's own padding
-    ([512, 12, 5], [512, 16, 5], np.float32, "thomas-pcr"),  # padding of the middle axis only
+    ([64, 48, 40], [72, 50, 40], np.float64, "lanes"),       # pitches multiples of 16 B
+    ([64, 48, 40], [67, 48, 40], np.float64, "sys_per_wg"),  # 536-byte rows
+    ([128, 30, 20], [132, 33, 20], np.float32, "lanes"),
+    ([128, 30, 20], [130, 30, 20], np.float32, "sys_per_wg"),
+    ([256, 64], [260, 64], np.float64, "lanes"),
+    ([100, 37], [104, 37], np.float64, "sys_per_wg"),        # no whole partition of 100
+    ([30, 7, 3], [32, 7, 3], np.float64, "sys_per_wg"),      # the reference's own padding
+    ([512, 12, 5], [512, 16, 5], np.float32, "lanes"),       # padding of the middle axis only
 ]
 
 
@@ -79,7 +81,8 @@ def test_padded_matches_oracle_and_keeps_padding(dims, pads, dtype, xkind):
             assert err < tol
             assert np.all(xp[mask] == dtype(SENTINEL))
             if sd == 0:
-                assert cfg.startswith(xkind), cfg
+                assert cfg.startswith("thomas-pcr") and xkind in cfg, cfg
+                assert nbytes == 0
 
 
 @pytest.mark.parametrize("dims,pads,dtype", [

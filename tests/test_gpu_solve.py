@@ -301,3 +301,36 @@ def test_streams_and_handles():
             P.solve(*t[:5], t[5], dims, 1, handle=h)
         s.synchronize()
         assert relerr(t[5].cpu().numpy(), ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("n", [1025, 1100, 3000, 4096, 8192])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_long_x_lines_on_chip(n, dtype):
+    """Contiguous lines longer than 1024 rows: the LDS-staged kernel with more
+    than 64 lanes per system (separators exchanged through LDS across the
+    workgroup), not the one-lane kernel with its scratch passes; the same
+    lines through the strided partitioned kernel at stride 1 (what a padded
+    layout whose pitch breaks 16-byte alignment runs) via "thomas-pcr"."""
+    dims = [n, 9, 3]
+    print(f"long x len={n} {np.dtype(dtype).name}", flush=True)
+    with P.Handle() as h:
+        check(gen(dims, n, dtype), dims, 0, h)
+        cfg = h.kernel_config(0)
+        assert P.kernel_that_ran(0) == "transpose"
+        print(f"  {cfg}", flush=True)
+        assert cfg.startswith("transpose rows=") and int(cfg.split("lanes=")[1]) > 64, cfg
+        h.set_algo(0, "thomas-pcr")
+        check(gen(dims, n + 1, dtype), dims, 0, h)
+        cfg = h.kernel_config(0)
+        print(f"  {cfg}", flush=True)
+        assert cfg.startswith("thomas-pcr rows=") and "sys_per_wg" in cfg, cfg
+
+
+def test_long_x_lines_full_batch():
+    """8192-row x lines, 512 of them, FP64, against the oracle; and one
+    8192-row system alone."""
+    for dims in ([8192, 64, 8], [8192]):
+        print(f"long x dims={dims}", flush=True)
+        with P.Handle() as h:
+            check(gen(dims, 3), dims, 0, h)
+            assert h.kernel_config(0) == "transpose rows=16 lanes=512", h.kernel_config(0)
