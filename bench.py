@@ -624,24 +624,37 @@ def workload_c5(P, torch, W, h, check, steps=100, cpu_steps=2, max_refine=5, tol
             res[mode]["refine_steps_max"] = refine
     # structured coefficients: the same operators generated in the kernels
     # from one scale per line (16 B per row FP64, 8 B FP32, against 56 / 36)
+    # ("stencil_ir": FP32 solves refined against the FP64 operator, so its
+    # answer is the FP64 operator's, not the FP32-stored one's)
     scales = W.hyperdiffusion_scales_torch(dims, 7)
-    for mode, sc in (("stencil_fp64", scales), ("stencil_fp32", [t.float() for t in scales])):
-        x = b.clone() if mode == "stencil_fp64" else b.float()
-        with P.AdiStencil(dims, sc, W.HYPERDIFFUSION_WEIGHTS, 1.0, handle=h) as adi:
+    for mode, sc in (("stencil_fp64", scales), ("stencil_fp32", [t.float() for t in scales]),
+                     ("stencil_ir", scales)):
+        x = b.float() if mode == "stencil_fp32" else b.clone()
+        with P.AdiStencil(dims, sc, W.HYPERDIFFUSION_WEIGHTS, 1.0, handle=h,
+                          refine=(mode == "stencil_ir")) as adi:
+            if mode == "stencil_ir":
+                adi.set_refine(max_refine, tol)
             ms = graph_ms_per_step(torch, adi, x, x.clone(), steps)
+            if mode == "stencil_ir":
+                refine = adi.refine_steps()
             x.copy_(b)
             adi.capture(x, steps)
             adi.replay(1)
             torch.cuda.synchronize()
             cfg = [adi.config(k) for k in range(3)]
         res[mode] = {"ms_per_step": ms, "grows": 3 * n / (ms * 1e-3) / 1e9, "kernels": cfg,
-                     "bytes_per_row": 16 if mode == "stencil_fp64" else 8,
+                     "bytes_per_row": 8 if mode == "stencil_fp32" else 16,
                      "max_rel_err_vs_assembled_fp64_after_steps": float(
                          ((x.double() - out["fp64"]).abs().max() /
                           out["fp64"].abs().max()).item())}
+        if mode == "stencil_ir":
+            res[mode]["refine_steps_max"] = refine
+            out["stencil_ir"] = x.clone()
         del x
     del scales
     res["stencil_fp64_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["stencil_fp64"]["ms_per_step"]
+    res["stencil_ir_speedup_vs_stencil_fp64"] = (res["stencil_fp64"]["ms_per_step"] /
+                                                 res["stencil_ir"]["ms_per_step"])
     # the refinement run solves the FP32-stored operator: its solver error after
     # all steps is measured against an FP64 run of that same (widened) operator
     xw = b.clone()
@@ -676,6 +689,17 @@ def workload_c5(P, torch, W, h, check, steps=100, cpu_steps=2, max_refine=5, tol
         cpu_s = time.perf_counter() - t0
         del c64
         res["fp64_max_rel_err_vs_cpu_fp64"] = max_rel(x.cpu().numpy(), xc)
+        # the structured refinement run against the same CPU run of the FP64
+        # operator (its residual uses that operator)
+        scales = W.hyperdiffusion_scales_torch(dims, 7)
+        with P.AdiStencil(dims, scales, W.HYPERDIFFUSION_WEIGHTS, 1.0, handle=h,
+                          refine=True) as adi:
+            adi.set_refine(max_refine, tol)
+            xs = b.clone()
+            adi.step(xs, cpu_steps)
+            torch.cuda.synchronize()
+        res["stencil_ir_max_rel_err_vs_cpu_fp64"] = max_rel(xs.cpu().numpy(), xc)
+        del scales, xs
         c32w = [[a.double().cpu().numpy() for a in cs] for cs in sets32]
         xw = b.cpu().numpy()
         for _ in range(cpu_steps):

@@ -323,7 +323,12 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t handle,
  * entry of scales[d].  scales: ndim device arrays, direction d's holding one
  * value per line laid out over the remaining axes in the standard order
  * (fastest first), copied at create; weights, id_weights on the host.
- *   mode 0: FP64 field and scales; mode 1: FP32 both.
+ *   mode 0: FP64 field and scales; mode 1: FP32 both; mode 2: FP64 field
+ *   and scales, every solve in FP32 with FP64 refinement whose residual
+ *   uses the FP64 operator, so the result is the FP64 operator's solution
+ *   (refined to tol 1e-15, at most 5 corrections, unless
+ *   pentadsolver_adi_set_refine says otherwise; lengths must split into 2..64
+ *   chunks of 4, 6, 8 or 10 rows).
  * The kernels generate the diagonals from the scales, so a step streams only
  * the field (16 B per row and direction in FP64).  Every length must split
  * into 2..64 chunks of 4..16 rows.  Step, capture, replay, timing and
@@ -349,8 +354,14 @@ pentadStatus_t pentadsolver_adi_graph_launch(pentadsolver_adi_t adi, int reps);
  * reference's apps/adi_cuda.cu:216-256. */
 pentadStatus_t pentadsolver_adi_direction_ms(pentadsolver_adi_t adi, void *x,
                                              int reps, float *ms);
+/* Stopping rule of the refinement modes (pentadsolver_adi_create mode 2,
+ * pentadsolver_adi_create_stencil mode 2): at most max_refine corrections,
+ * stop at estimated relative forward error tol (as in
+ * pentadsolver_DS_gpsv_batch_ir).  Applies to later steps and captures. */
+pentadStatus_t pentadsolver_adi_set_refine(pentadsolver_adi_t adi, int max_refine,
+                                           double tol);
 /* Largest number of refinement steps any line took since the last call
- * (mode 2; synchronises the stream), then resets the counter. */
+ * (refinement modes; synchronises the stream), then resets the counter. */
 int pentadsolver_adi_refine_steps(pentadsolver_adi_t adi);
 /* Kernel configuration used along direction dir. */
 const char *pentadsolver_adi_config(pentadsolver_adi_t adi, int dir);

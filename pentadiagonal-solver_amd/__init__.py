@@ -75,6 +75,7 @@ _SIGNATURES = {
     "pentadsolver_adi_graph_capture": (_I, [_P, _P, _I]),
     "pentadsolver_adi_graph_launch": (_I, [_P, _I]),
     "pentadsolver_adi_refine_steps": (_I, [_P]),
+    "pentadsolver_adi_set_refine": (_I, [_P, _I, ctypes.c_double]),
     "pentadsolver_adi_direction_ms": (_I, [_P, _P, _I, ctypes.POINTER(ctypes.c_float)]),
     "pentadsolver_adi_config": (ctypes.c_char_p, [_P, _I]),
     "pentadsolver_adi_destroy": (None, [ctypes.POINTER(_P)]),
@@ -468,6 +469,9 @@ class Adi:
         _check(load_library().pentadsolver_adi_direction_ms(self._a, _ptr(x), int(reps), ms))
         return [float(v) for v in ms[: self.ndim]]
 
+    def set_refine(self, max_refine: int, tol: float) -> None:
+        _check(load_library().pentadsolver_adi_set_refine(self._a, int(max_refine), float(tol)))
+
     def refine_steps(self) -> int:
         return load_library().pentadsolver_adi_refine_steps(self._a)
 
@@ -500,11 +504,13 @@ class AdiStencil(Adi):
     past the ends of the line dropped, with s the line's entry of scales[d]:
     a device tensor with one value per line over the remaining axes in the
     standard order.  float64 scales give the FP64 driver, float32 the FP32
-    one (field of the same dtype).  weights: ndim sequences of 5 floats (or
-    one for every direction), id_weights: ndim floats (or one).
+    one (field of the same dtype); refine=True with float64 scales solves
+    every line in FP32 with FP64 refinement against the FP64 operator (mode
+    2).  weights: ndim sequences of 5 floats (or one for every direction),
+    id_weights: ndim floats (or one).
     """
 
-    def __init__(self, dims, scales, weights, id_weights=1.0, handle=None):
+    def __init__(self, dims, scales, weights, id_weights=1.0, handle=None, refine=False):
         lib = load_library()
         self._a = _P()
         self.ndim = len(dims)
@@ -522,8 +528,11 @@ class AdiStencil(Adi):
         wa = (ctypes.c_double * (5 * self.ndim))(*[float(v) for r in w for v in r])
         ia = (ctypes.c_double * self.ndim)(*ids)
         ptrs = (_P * self.ndim)(*[_ptr(t) for t in scales])
+        if refine and eb != 8:
+            raise TypeError("the refinement mode takes float64 scales")
+        mode = 2 if refine else (0 if eb == 8 else 1)
         _check(lib.pentadsolver_adi_create_stencil(_hval(handle), ctypes.byref(self._a),
-                                                   0 if eb == 8 else 1, self.ndim,
+                                                   mode, self.ndim,
                                                    _dims_arg(dims), ptrs, wa, ia))
 
 

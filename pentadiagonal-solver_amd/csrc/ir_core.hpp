@@ -19,6 +19,11 @@
 // (8 + 8 B/row); every refinement step runs on chip.  The FP64 result solves
 // widen(A32) x = b, the system the FP64 oracle solves.
 //
+// The residual's operator comes from the accessor's FP64 values (s64 ..
+// w64): the FP32 coefficients widened for stored coefficients, or an FP64
+// operator of its own (structured coefficients generated from an FP64 scale,
+// whose solution the refinement then converges to).
+//
 // Register budget: the coefficients are reached through an accessor (Cf),
 // which the x-direction kernel backs with LDS so that the per-lane state is
 // the FP64 iterate, the LU factors (du2, 1/pivot, two spikes) and the PCR
@@ -42,6 +47,12 @@ template <int M> struct RegCoef {
   __device__ __forceinline__ float d(int i) const { return d_[i]; }
   __device__ __forceinline__ float u(int i) const { return u_[i]; }
   __device__ __forceinline__ float w(int i) const { return w_[i]; }
+  // the operator of the FP64 residual: the FP32 coefficients widened
+  __device__ __forceinline__ double s64(int i) const { return (double)s_[i]; }
+  __device__ __forceinline__ double l64(int i) const { return (double)l_[i]; }
+  __device__ __forceinline__ double d64(int i) const { return (double)d_[i]; }
+  __device__ __forceinline__ double u64(int i) const { return (double)u_[i]; }
+  __device__ __forceinline__ double w64(int i) const { return (double)w_[i]; }
   __device__ __forceinline__ void refresh() const {}
 };
 
@@ -54,6 +65,11 @@ template <int M> struct LdsCoef {
   __device__ __forceinline__ float d(int i) const { return at(2, i); }
   __device__ __forceinline__ float u(int i) const { return at(3, i); }
   __device__ __forceinline__ float w(int i) const { return at(4, i); }
+  __device__ __forceinline__ double s64(int i) const { return (double)s(i); }
+  __device__ __forceinline__ double l64(int i) const { return (double)l(i); }
+  __device__ __forceinline__ double d64(int i) const { return (double)d(i); }
+  __device__ __forceinline__ double u64(int i) const { return (double)u(i); }
+  __device__ __forceinline__ double w64(int i) const { return (double)w(i); }
   // Forget earlier loads (compiler-level memory clobber, no instruction):
   // without it the compiler keeps every coefficient it has read in
   // registers, which is what keeping them in LDS is meant to avoid.
@@ -418,11 +434,11 @@ __device__ __forceinline__ int ir_solve(const Cf &A, const Bv &b, X &x,
       const double xm1 = (i >= 1) ? x[i - 1] : hp[3];
       const double xp1 = (i + 1 < M) ? x[i + 1] : hn[0];
       const double xp2 = (i + 2 < M) ? x[i + 2] : hn[i + 2 - M];
-      double acc = (double)A.d(i) * x[i];
-      acc += (double)A.s(i) * xm2;
-      acc += (double)A.l(i) * xm1;
-      acc += (double)A.u(i) * xp1;
-      acc += (double)A.w(i) * xp2;
+      double acc = A.d64(i) * x[i];
+      acc += A.s64(i) * xm2;
+      acc += A.l64(i) * xm1;
+      acc += A.u64(i) * xp1;
+      acc += A.w64(i) * xp2;
       f[i] = (float)(b(i) - acc);
     }
     ir_apply<M, MAXR>(F, A, f, p, P, ex);

@@ -21,6 +21,7 @@
 //
 // No reference counterpart: the reference assembles and reads five
 // diagonal arrays for every solve (as apps/preproc.hpp:248 builds them).
+#include "ir_core.hpp"
 #include "partition_core.hpp"
 #include "partition_exchange.hpp"
 #include "rows_io.hpp"
@@ -28,6 +29,58 @@
 #include "pentad_internal.hpp"
 
 namespace pentad {
+
+// Coefficients of one chunk of a structured line for the refinement solve:
+// the FP32 operator (the FP64 diagonals rounded) for the bulk solve, the
+// FP64 operator (generated from the FP64 scale) for the residual, so the
+// refinement converges to the FP64 operator's solution.  Chunks are whole
+// (len = M x lanes), so only the first chunk's lower couplings of rows 0, 1
+// and the last chunk's upper couplings of rows M-2, M-1 reach past an end
+// (zero); every other row has all five values.  Five values per line
+// instead of five per row: no coefficient array occupies registers.
+template <int M> struct StencilIrCoef {
+  float a, b, c, e, f;
+  double a64, b64, c64, e64, f64;
+  bool first, last;
+  template <typename T>
+  __device__ __forceinline__ T lo(int i, int from, T v) const {
+    return (i >= from || !first) ? v : T(0);
+  }
+  template <typename T>
+  __device__ __forceinline__ T hi(int i, int upto, T v) const {
+    return (i + upto < M || !last) ? v : T(0);
+  }
+  __device__ __forceinline__ float s(int i) const { return lo(i, 2, a); }
+  __device__ __forceinline__ float l(int i) const { return lo(i, 1, b); }
+  __device__ __forceinline__ float d(int) const { return c; }
+  __device__ __forceinline__ float u(int i) const { return hi(i, 1, e); }
+  __device__ __forceinline__ float w(int i) const { return hi(i, 2, f); }
+  __device__ __forceinline__ double s64(int i) const { return lo(i, 2, a64); }
+  __device__ __forceinline__ double l64(int i) const { return lo(i, 1, b64); }
+  __device__ __forceinline__ double d64(int) const { return c64; }
+  __device__ __forceinline__ double u64(int i) const { return hi(i, 1, e64); }
+  __device__ __forceinline__ double w64(int i) const { return hi(i, 2, f64); }
+  __device__ __forceinline__ void refresh() const {}
+};
+
+template <int M>
+__device__ __forceinline__ StencilIrCoef<M> stencil_ir_coef(const Stencil<double> &st,
+                                                           double sc, bool first, bool last) {
+  StencilIrCoef<M> A;
+  A.a64 = sc * st.w[0];
+  A.b64 = sc * st.w[1];
+  A.c64 = st.id + sc * st.w[2];
+  A.e64 = sc * st.w[3];
+  A.f64 = sc * st.w[4];
+  A.a = (float)A.a64;
+  A.b = (float)A.b64;
+  A.c = (float)A.c64;
+  A.e = (float)A.e64;
+  A.f = (float)A.f64;
+  A.first = first;
+  A.last = last;
+  return A;
+}
 
 // rows r0 .. r0+M-1 of a line of length len with scale s; rows past the end
 // (padded chunks) are identity rows
@@ -136,10 +189,90 @@ __global__ void __launch_bounds__(MAXT)
   }
 }
 
+// FP32 solve + FP64 refinement over structured coefficients, x-like lines
+// with the rotated store (the ADI driver): Q passes of BLOCK >> R systems
+// into one tile whose rows are whole 128-byte lines, as k_ir_xq; HBM traffic
+// per row: 8 B in, 8 B out (plus one scale per line).
+template <int M, int R, int Q, int BLOCK>
+__global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
+    k_stencil_ir_x(Stencil<double> st, const double *b, double *out, int64_t n_sys,
+                   int64_t len, int max_steps, double tol, int *__restrict__ steps_out,
+                   XMap map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int L = 1 << R;
+  constexpr int s0shift = (BLOCK == 512 ? 9 : 8) - R; // systems per pass
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int sl = (int)threadIdx.x >> R;
+  const int p = (int)(threadIdx.x & (L - 1));
+  const int lane = threadIdx.x & 63;
+  WaveEx ex{lane, p, L};
+  int kmax = 0;
+#pragma unroll 1
+  for (int q = 0; q < Q; ++q) {
+    const int64_t sys = ((blk * Q + q) << s0shift) + sl;
+    const int64_t sc = sys < n_sys ? sys : n_sys - 1;
+    const int64_t slo = map.lo(sc), shi = map.hi(sc);
+    const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
+    const StencilIrCoef<M> A =
+        stencil_ir_coef<M>(st, st.scale[slo * st.sa + shi * st.sb], p == 0, p == L - 1);
+    const MemRhs bv{b + off, 1};
+    double xs[M];
+    const int k = ir_solve<M, ir_stored_rounds(R)>(A, bv, xs, p, L, ex, max_steps, tol);
+    kmax = k > kmax ? k : kmax;
+    tile_put<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, Q << s0shift,
+                        (q << s0shift) + sl);
+  }
+  __syncthreads();
+  constexpr int sshift = s0shift + (Q == 4 ? 2 : (Q == 2 ? 1 : 0));
+  tile_flush<double, M>(reinterpret_cast<const double *>(smem_raw), sshift, (int)len, out,
+                        n_sys, blk << sshift, map);
+  // largest step count over the launch (read first: the atomic is rarely needed)
+  if (lane == 0 && steps_out != nullptr) {
+    const int cur = __hip_atomic_load(steps_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (kmax > cur) atomicMax(steps_out, kmax);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 namespace {
+
+template <int M, int R>
+hipError_t go_stencil_ir(const Stencil<double> &stc, const double *b, double *out,
+                         int64_t n_sys, int64_t len, const XMap &map, int max_steps, double tol,
+                         int *steps, hipStream_t st) {
+  // 10-row chunks: a 16-system tile of 640-row lines is 82 KB, one
+  // 512-thread workgroup per CU (as k_ir_xq)
+  constexpr int B = M == 10 ? 512 : 256;
+  constexpr int run = (B >> R) * 8;
+  constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
+  const int64_t per_wg = (int64_t)(B >> R) * Q;
+  const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+  const size_t smem = rotated_tile_bytes<double>(B * Q, M, len);
+  const hipError_t e = hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&k_stencil_ir_x<M, R, Q, B>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_stencil_ir_x<M, R, Q, B>), dim3((unsigned)grid), dim3(B), smem, st, stc,
+                     b, out, n_sys, len, max_steps, tol, steps, map);
+  return hipGetLastError();
+}
+
+template <int M>
+hipError_t go_stencil_ir_r(const Stencil<double> &stc, const double *b, double *out,
+                           int64_t n_sys, int64_t len, const XMap &map, int lanes,
+                           int max_steps, double tol, int *steps, hipStream_t st) {
+  switch (lanes) {
+  case 2: return go_stencil_ir<M, 1>(stc, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  case 4: return go_stencil_ir<M, 2>(stc, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  case 8: return go_stencil_ir<M, 3>(stc, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  case 16: return go_stencil_ir<M, 4>(stc, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  case 32: return go_stencil_ir<M, 5>(stc, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  case 64: return go_stencil_ir<M, 6>(stc, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  }
+  return hipErrorInvalidValue;
+}
 
 template <typename T, int M>
 hipError_t go_sx(const Stencil<T> &stc, const T *b, T *out, bool rot, int64_t n_sys,
@@ -240,6 +373,21 @@ hipError_t launch_stencil_yz(const Stencil<T> &stc, T *x, const Lines &ln, int r
   case 8: return go_syz<T, 8>(stc, x, ln, sys_per_wg, st);
   case 12: return go_syz<T, 12>(stc, x, ln, sys_per_wg, st);
   case 16: return go_syz<T, 16>(stc, x, ln, sys_per_wg, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stencil_ir_x_map(const Stencil<double> &stc, const double *b, double *out,
+                                   int64_t n_sys, int64_t len, const XMap &map, int rows,
+                                   int max_steps, double tol, int *steps, hipStream_t st) {
+  if (!ir_x_supported(rows, len)) return hipErrorInvalidValue;
+  const int lanes = (int)(len / rows);
+  switch (rows) {
+  case 4: return go_stencil_ir_r<4>(stc, b, out, n_sys, len, map, lanes, max_steps, tol, steps, st);
+  case 6: return go_stencil_ir_r<6>(stc, b, out, n_sys, len, map, lanes, max_steps, tol, steps, st);
+  case 8: return go_stencil_ir_r<8>(stc, b, out, n_sys, len, map, lanes, max_steps, tol, steps, st);
+  case 10:
+    return go_stencil_ir_r<10>(stc, b, out, n_sys, len, map, lanes, max_steps, tol, steps, st);
   }
   return hipErrorInvalidValue;
 }

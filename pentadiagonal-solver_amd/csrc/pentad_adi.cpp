@@ -79,7 +79,7 @@ struct Adi {
   void *coef[3][5] = {};
   void *coef_mem = nullptr;
   void *tab[3] = {}; // shared-fact factor tables (modes 3, 4), in coef_mem
-  void *scale[3] = {}; // structured coefficients (modes 5, 6), in coef_mem
+  void *scale[3] = {}; // structured coefficients (modes 5, 6, 7), in coef_mem
   double sw[3][5] = {};
   double sid[3] = {1.0, 1.0, 1.0};
   Order corder[3]; // coefficient layout per direction
@@ -162,7 +162,7 @@ hipError_t solve_dir(Adi *A, int d, const void *in, void *out, bool first_step,
   const Plan &pl = A->plan[d];
   const int64_t len = A->dims[d], nsys = A->n / len;
   const XMap map = make_map(A, d, first_step, last_step);
-  if (A->mode == 5 || A->mode == 6) {
+  if (A->mode >= 5) {
     // scale of system (lo, hi): the caller lays out direction d's scales over
     // the remaining axes in the standard order (fastest first)
     const int ndim = A->ndim, X = (d + 1) % ndim, Y = (ndim == 3) ? (d + 2) % 3 : -1;
@@ -170,6 +170,16 @@ hipError_t solve_dir(Adi *A, int d, const void *in, void *out, bool first_step,
     if (Y >= 0) {
       if (X < Y) sb = A->dims[X];
       else { sa = A->dims[Y]; sb = 1; }
+    }
+    if (A->mode == 7) { // FP32 solve + FP64 refinement against the FP64 operator
+      Stencil<double> stc;
+      stc.scale = (const double *)A->scale[d];
+      stc.sa = sa;
+      stc.sb = sb;
+      for (int k = 0; k < 5; ++k) stc.w[k] = A->sw[d][k];
+      stc.id = A->sid[d];
+      return launch_stencil_ir_x_map(stc, (const double *)in, (double *)out, nsys, len, map,
+                                     pl.rows, A->max_refine, A->tol, A->d_steps, st);
     }
     if (A->mode == 5) {
       Stencil<double> stc;
@@ -453,7 +463,7 @@ pentadStatus_t pentadsolver_adi_create_stencil(pentadsolver_handle_t h,
                                                const int *dims, const void *const *scales,
                                                const double *weights,
                                                const double *id_weights) {
-  if (!out || !dims || !scales || !weights || !id_weights || (mode != 0 && mode != 1) ||
+  if (!out || !dims || !scales || !weights || !id_weights || mode < 0 || mode > 2 ||
       ndim < 2 || ndim > 3)
     return PENTAD_STATUS_INVALID_VALUE;
   for (int i = 0; i < ndim; ++i)
@@ -463,14 +473,18 @@ pentadStatus_t pentadsolver_adi_create_stencil(pentadsolver_handle_t h,
   Context *c = ctx_of(h);
   Adi *A = new Adi();
   A->c = c;
-  A->mode = mode == 0 ? 5 : 6;
+  A->mode = mode == 0 ? 5 : (mode == 1 ? 6 : 7);
+  if (mode == 2) { // refine every solve to the limit of FP64 by default
+    A->max_refine = 5;
+    A->tol = 1e-15;
+  }
   A->ndim = ndim;
   A->n = 1;
   for (int i = 0; i < ndim; ++i) {
     A->dims[i] = dims[i];
     A->n *= dims[i];
   }
-  A->cbytes = A->fbytes = (mode == 0) ? 8 : 4;
+  A->cbytes = A->fbytes = (mode == 1) ? 4 : 8;
   A->rotating = true;
   hipStream_t st = c->stream;
   auto fail = [&](pentadStatus_t s) {
@@ -483,7 +497,19 @@ pentadStatus_t pentadsolver_adi_create_stencil(pentadsolver_handle_t h,
     ln.dir = 0;
     ln.len = A->dims[d];
     ln.n_sys = A->n / ln.len;
-    if (!stencil_plan_x(A->fbytes, ln.len, A->plan[d])) {
+    bool planned = false;
+    if (A->mode == 7) { // the refinement kernels' chunk sizes
+      for (int r : {8, 6, 4, 10})
+        if (!planned && ir_x_supported(r, ln.len)) {
+          A->plan[d].kind = Plan::IrX;
+          A->plan[d].rows = r;
+          A->plan[d].lanes = (int)(ln.len / r);
+          planned = true;
+        }
+    } else {
+      planned = stencil_plan_x(A->fbytes, ln.len, A->plan[d]);
+    }
+    if (!planned) {
       std::fprintf(stderr, "[penta] adi stencil: no partitioned kernel for length %lld\n",
                    (long long)ln.len);
       return fail(PENTAD_STATUS_INVALID_VALUE);
@@ -512,7 +538,7 @@ pentadStatus_t pentadsolver_adi_create_stencil(pentadsolver_handle_t h,
     if (hipMemcpyAsync(A->scale[d], scales[d], (size_t)(A->n / A->dims[d]) * A->fbytes,
                        hipMemcpyDeviceToDevice, st) != hipSuccess)
       return fail(PENTAD_STATUS_EXECUTION_FAILED);
-    A->cfg[d] = "stencil " + plan_config(A->plan[d]);
+    A->cfg[d] = "stencil " + plan_config(A->plan[d]); // "stencil ir ..." in mode 2
   }
   if (hipStreamSynchronize(st) != hipSuccess) return fail(PENTAD_STATUS_EXECUTION_FAILED);
   *out = A;
@@ -535,7 +561,7 @@ pentadStatus_t pentadsolver_adi_step(pentadsolver_adi_t adi, void *x, int nsteps
     for (int d = 0; d < A->ndim; ++d)
       note_kernel(A->c, d,
                   A->mode == 2 ? "ir"
-                               : (A->mode >= 5 ? "stencil"
+                               : (A->mode >= 5 ? (A->mode == 7 ? "stencil ir" : "stencil")
                                                : (A->mode >= 3 ? "shared-fact" : "thomas-pcr")),
                   A->cfg[d]);
   return PENTAD_STATUS_SUCCESS;
@@ -621,6 +647,15 @@ pentadStatus_t pentadsolver_adi_direction_ms(pentadsolver_adi_t adi, void *x,
   for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]);
   for (int d = 0; d < 3; ++d) ms[d] = (d < A->ndim) ? (float)(acc[d] / reps) : 0.f;
   return status;
+}
+
+pentadStatus_t pentadsolver_adi_set_refine(pentadsolver_adi_t adi, int max_refine,
+                                           double tol) {
+  Adi *A = static_cast<Adi *>(adi);
+  if (!A || max_refine < 0 || !(tol >= 0.0)) return PENTAD_STATUS_INVALID_VALUE;
+  A->max_refine = max_refine;
+  A->tol = tol;
+  return PENTAD_STATUS_SUCCESS;
 }
 
 int pentadsolver_adi_refine_steps(pentadsolver_adi_t adi) {

@@ -189,6 +189,12 @@ hipError_t launch_stencil_x_map(const Stencil<T> &stc, const T *b, T *out, bool 
                                 int64_t n_sys, int64_t len, const XMap &map, int rows,
                                 int lanes, hipStream_t st);
 bool stencil_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
+// FP32 solve + FP64 refinement over FP64 structured coefficients, x-like
+// lines with the rotated store (rows 4, 6, 8, 10 with lanes = len / rows a
+// power of two <= 64): converges to the FP64 operator's solution
+hipError_t launch_stencil_ir_x_map(const Stencil<double> &stc, const double *b, double *out,
+                                   int64_t n_sys, int64_t len, const XMap &map, int rows,
+                                   int max_steps, double tol, int *steps, hipStream_t st);
 template <typename T>
 hipError_t launch_stencil_yz(const Stencil<T> &stc, T *x, const Lines &ln, int rows,
                              int sys_per_wg, hipStream_t st);

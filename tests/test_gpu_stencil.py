@@ -188,3 +188,70 @@ def test_adi_stencil_fp32(dims):
     err = float(((x - xg).abs().max() / xg.abs().max()).item())
     print(f"fp32 stencil ADI vs assembled FP32 ADI {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("dims", [[64, 32, 48], [384, 256, 12], [24, 16, 640], [96, 80]])
+@pytest.mark.parametrize("wset", ["hyper", "skew"])
+def test_adi_stencil_refinement_matches_fp64_operator(dims, wset):
+    """Structured coefficients in FP32 + FP64 refinement (stencil mode 2):
+    the residual uses the FP64 operator, so two steps match the oracle's
+    FP64 solves of the FP64 operator (assembled on the host) to <= 1e-12,
+    which the FP32-stored operator alone misses by about 1e-6; graph replay
+    bit-identical to eager steps."""
+    weights, idw = HYPER if wset == "hyper" else SKEW
+    sc = adi_scales(dims, 61, torch.float64)
+    rng = np.random.default_rng(62)
+    b = torch.from_numpy(rng.standard_normal(int(np.prod(dims)))).cuda()
+    print(f"adi stencil refinement dims={dims} {wset}", flush=True)
+    x = b.clone()
+    with P.AdiStencil(dims, sc, weights, idw, refine=True) as adi:
+        adi.step(x, 2)
+        torch.cuda.synchronize()
+        steps = adi.refine_steps()
+        cfg = [adi.config(k) for k in range(len(dims))]
+        xg = b.clone()
+        adi.capture(xg, 2)
+        adi.replay(1)
+        torch.cuda.synchronize()
+        assert torch.equal(xg, x)
+    ref = b.cpu().numpy().copy()
+    for _ in range(2):
+        for d in range(len(dims)):
+            a = assemble(dims, d, sc[d].cpu().numpy().reshape(line_shape(dims, d)), weights, idw)
+            ref = O.solve(*a, ref, dims, d)
+    err = relerr(x.cpu().numpy(), ref)
+    print(f"  err vs FP64 operator {err:.2e} steps {steps} {cfg}", flush=True)
+    assert all(c.startswith("stencil ir thomas-pcr") for c in cfg), cfg
+    assert 1 <= steps <= 5
+    assert err < 1e-12
+
+
+def test_adi_stencil_refinement_set_refine():
+    """No correction at all leaves the FP32 solve of the FP32-rounded
+    operator (error well above 1e-12); the default stopping rule reaches
+    the FP64 operator."""
+    dims = [128, 64, 32]
+    weights, idw = HYPER
+    sc = adi_scales(dims, 71, torch.float64)
+    b = torch.from_numpy(np.random.default_rng(72).standard_normal(int(np.prod(dims)))).cuda()
+    ref = b.cpu().numpy().copy()
+    for d in range(3):
+        a = assemble(dims, d, sc[d].cpu().numpy().reshape(line_shape(dims, d)), weights, idw)
+        ref = O.solve(*a, ref, dims, d)
+    with P.AdiStencil(dims, sc, weights, idw, refine=True) as adi:
+        x = b.clone()
+        adi.set_refine(0, 0.0)
+        adi.step(x, 1)
+        torch.cuda.synchronize()
+        e0 = relerr(x.cpu().numpy(), ref)
+        assert adi.refine_steps() == 0
+        x = b.clone()
+        adi.set_refine(5, 1e-15)
+        adi.step(x, 1)
+        torch.cuda.synchronize()
+        e1 = relerr(x.cpu().numpy(), ref)
+    print(f"no correction {e0:.2e}, refined {e1:.2e}", flush=True)
+    assert 1e-10 < e0 < 1e-4
+    assert e1 < 1e-12
+    with pytest.raises(TypeError):
+        P.AdiStencil(dims, [t.float() for t in sc], weights, idw, refine=True)
