@@ -47,19 +47,23 @@ __device__ __forceinline__ double cav_c(double ia, double ib, double half_au, do
   return half_au * (ia + ib) * h2;
 }
 
-__device__ __forceinline__ void cavity_point(const Cavity &c, int i, int j, double &ds,
-                                             double &dl, double &d, double &du, double &dw,
-                                             double &rhs) {
+// The same with the five reciprocals given: 1/uv of the point (ic) and of
+// its east, west, north and south neighbours.  Callers walking along a line
+// pass each reciprocal to the next point (it is that point's west or north
+// one), three divisions per point instead of five; division is correctly
+// rounded, so the values are the same bit for bit.
+__device__ __forceinline__ void cavity_point_inv(const Cavity &c, int i, int j, double ic,
+                                                 double ie, double iw, double in_, double is_,
+                                                 double &ds, double &dl, double &d, double &du,
+                                                 double &dw, double &rhs) {
 #pragma clang fp contract(off)
   const int lu = c.nx + 1, lv = c.nx + 2;
-  const double *uv = c.uv + (int64_t)i * lv + j;
   const double half_au = 0.5 * c.alpha_u;
   const double dx2 = c.dx * c.dx, dy2 = c.dy * c.dy;
-  const double ic = 1.0 / uv[0];
-  const double E = cav_c(ic, 1.0 / uv[1], half_au, dy2);
-  const double W = cav_c(ic, 1.0 / uv[-1], half_au, dy2);
-  const double N = cav_c(ic, 1.0 / uv[-lv], half_au, dx2);
-  const double S = cav_c(ic, 1.0 / uv[lv], half_au, dx2);
+  const double E = cav_c(ic, ie, half_au, dy2);
+  const double W = cav_c(ic, iw, half_au, dy2);
+  const double N = cav_c(ic, in_, half_au, dx2);
+  const double S = cav_c(ic, is_, half_au, dx2);
   const bool top = i == 1, bot = i == c.ny, left = j == 1, right = j == c.nx;
   double s = N, l = W, u = E, w = S;
   if (top) {
@@ -89,6 +93,16 @@ __device__ __forceinline__ void cavity_point(const Cavity &c, int i, int j, doub
   rhs = __builtin_fma(p, c.dy, -(q * c.dx));
 }
 
+__device__ __forceinline__ void cavity_point(const Cavity &c, int i, int j, double &ds,
+                                             double &dl, double &d, double &du, double &dw,
+                                             double &rhs) {
+#pragma clang fp contract(off)
+  const int lv = c.nx + 2;
+  const double *uv = c.uv + (int64_t)i * lv + j;
+  cavity_point_inv(c, i, j, 1.0 / uv[0], 1.0 / uv[1], 1.0 / uv[-1], 1.0 / uv[-lv],
+                   1.0 / uv[lv], ds, dl, d, du, dw, rhs);
+}
+
 // one thread per grid point; point (i, j) to (i-1) * nx_pad + (j-1)
 __global__ void __launch_bounds__(256) k_cavity_assemble(Cavity c, CavityOut o) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,19 +126,38 @@ template <int M, bool LINE_X>
 __device__ __forceinline__ void cavity_rows(const Cavity &c, int fixed, int r0, double (&s)[M],
                                             double (&l)[M], double (&d)[M], double (&u)[M],
                                             double (&w)[M], double (&f)[M]) {
+#pragma clang fp contract(off)
   const int len = LINE_X ? c.nx : c.ny;
+  const int lv = c.nx + 2;
+  // 1/uv along the line: point r (grid index r + 1) needs line indices r,
+  // r + 1, r + 2, carried from point to point; indices past the far halo
+  // (rows past the end, whose values are discarded) are clamped to it
+  auto along = [&](int q) {
+    const int qc = q <= len + 1 ? q : len + 1;
+    return LINE_X ? c.uv[(int64_t)fixed * lv + qc] : c.uv[(int64_t)qc * lv + fixed];
+  };
+  double iw = 1.0 / along(r0), ic = 1.0 / along(r0 + 1);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const int r = r0 + k;
+    const double ie = 1.0 / along(r + 2);
     // rows past the end evaluate a real point (the line's last) and are
     // then overwritten by identity rows: no divergent branch around the
     // generator
     const int rc = r < len ? r : len - 1;
     double rhs;
-    if constexpr (LINE_X)
-      cavity_point(c, fixed, rc + 1, s[k], l[k], d[k], u[k], w[k], rhs);
-    else
-      cavity_point(c, rc + 1, fixed, s[k], l[k], d[k], u[k], w[k], rhs);
+    if constexpr (LINE_X) {
+      const double *uvp = c.uv + (int64_t)fixed * lv + (rc + 1);
+      cavity_point_inv(c, fixed, rc + 1, ic, ie, iw, 1.0 / uvp[-lv], 1.0 / uvp[lv], s[k], l[k],
+                       d[k], u[k], w[k], rhs);
+    } else {
+      const double *uvp = c.uv + (int64_t)(rc + 1) * lv + fixed;
+      // along a column the line's neighbours are north (before) and south
+      cavity_point_inv(c, rc + 1, fixed, ic, 1.0 / uvp[1], 1.0 / uvp[-1], iw, ie, s[k], l[k],
+                       d[k], u[k], w[k], rhs);
+    }
+    iw = ic;
+    ic = ie;
     if constexpr (LINE_X) f[k] = rhs;
     if (r < 2) s[k] = 0.0;
     if (r < 1) l[k] = 0.0;
