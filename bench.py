@@ -867,6 +867,13 @@ def main():
     dropin_ms, dropin_k = dropin_direction_ms(P, torch, c64, b, dims, h)
     log("bench: drop-in in-place mixed-precision direction solves")
     dropin_ir_ms, dropin_ir_k = dropin_direction_ms(P, torch, c32, b, dims, h, ir=True)
+    log("bench: drop-in FP64 direction solves with PENTA_AUTOTUNE=1 (tuned on first use)")
+    os.environ["PENTA_AUTOTUNE"] = "1"
+    try:
+        with P.Handle() as ht:
+            tuned_ms, tuned_k = dropin_direction_ms(P, torch, c64, b, dims, ht)
+    finally:
+        del os.environ["PENTA_AUTOTUNE"]
 
     check = {}
     c64_np = c32_np = b_np = None
@@ -949,6 +956,11 @@ def main():
                                 "grows": 3 * n / (sum(dropin_ms) * 1e-3) / 1e9},
         "dropin_ir_inplace": {"per_direction_ms": dropin_ir_ms, "kernels": dropin_ir_k,
                               "grows": 3 * n / (sum(dropin_ir_ms) * 1e-3) / 1e9},
+        "dropin_fp64_autotuned": {"per_direction_ms": tuned_ms, "kernels": tuned_k,
+                                  "grows": 3 * n / (sum(tuned_ms) * 1e-3) / 1e9,
+                                  "how": "fresh handle, PENTA_AUTOTUNE=1: the first solve of "
+                                         "each direction times every configuration on the "
+                                         "same arrays (untimed), later solves use the winner"},
     }
     if args.probe:
         result["probe_ir_byte_mix"] = probe

@@ -242,6 +242,10 @@ void pentadsolver_DS_gpsv_batch_ir(pentadsolver_handle_t handle,
  * restored before returning.  Synchronises the stream and allocates a
  * temporary copy of x.  Returns the number of configurations timed (< 0 on
  * error); report (may be NULL) receives one "config time" line each.
+ * With PENTA_AUTOTUNE=1 in the environment, pentadsolver_{D,S}_gpsv_batch
+ * runs this tuning itself the first time a (precision, direction, length)
+ * is solved with the automatic choice on a handle (that call synchronises
+ * and allocates once; later calls of the key are as before).
  * No reference counterpart (the reference uses fixed dispatch rules and
  * PENTA_PCR_LANES, src/cuda/singlenode/pentad_cuda.cu:613-697). */
 int pentadsolver_D_autotune(pentadsolver_handle_t handle, const double *ds,

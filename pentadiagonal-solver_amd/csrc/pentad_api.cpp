@@ -125,6 +125,11 @@ std::string env_algo(int dir) {
   return s;
 }
 
+bool autotune_env() {
+  const char *v = std::getenv("PENTA_AUTOTUNE");
+  return v != nullptr && v[0] == '1';
+}
+
 bool debug_launch() {
   const char *v = std::getenv("PENTA_DEBUG_LAUNCH");
   return v != nullptr && v[0] == '1';
@@ -742,6 +747,13 @@ template <typename T>
 pentadStatus_t gpsv_batch(Context *c, const T *ds, const T *dl, const T *d,
                           const T *du, const T *dw, T *x, const int *dims,
                           int ndim, int solvedim, void *buffer) {
+  // opt-in: the automatic choice of a new (precision, direction, length)
+  // tuned on these arrays first (synchronises and allocates a copy of x once
+  // per key; the solve below then runs the winner)
+  Lines ln;
+  if (autotune_env() && ds && dl && d && du && dw && x &&
+      decompose(dims, ndim, solvedim, &ln) == 0 && requested_algo(c, ln.dir) == "auto")
+    (void)autotune_on_first_use<T>(c, ds, dl, d, du, dw, x, dims, ndim, solvedim);
   return gpsv_batch_pitched<T>(c, ds, dl, d, du, dw, x, dims, nullptr, ndim, solvedim, buffer,
                                SIZE_MAX);
 }

@@ -84,6 +84,12 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
                                   const T *du, const T *dw, T *x, const int *dims,
                                   const int *pads, int ndim, int solvedim, void *buffer,
                                   size_t buffer_cap);
+// PENTA_AUTOTUNE=1: tune a (precision, direction, length) the first time it
+// is solved with the automatic choice (pentad_tune.cpp); true when it tuned
+bool autotune_env();
+template <typename T>
+bool autotune_on_first_use(Context *c, const T *ds, const T *dl, const T *d, const T *du,
+                           const T *dw, T *x, const int *dims, int ndim, int solvedim);
 // Device address of the fixed scratch array of the allocation-free solve on
 // device dev (looked up once per device; pentad_noalloc.hip).
 hipError_t noalloc_scratch(int dev, void **out);

@@ -251,6 +251,37 @@ int autotune_ir(Context *c, const float *ds, const float *dl, const float *d,
 }
 
 } // namespace
+
+// First solve of a key under PENTA_AUTOTUNE=1: tune that (precision,
+// direction, length) on the caller's arrays unless the handle already holds
+// an entry for it (x is left as it was).  Returns true when it tuned.  A key
+// with no candidate (unaligned arrays, for example) gets the one-lane entry,
+// which the planner ignores, so that it is not tried again.
+template <typename T>
+bool autotune_on_first_use(Context *c, const T *ds, const T *dl, const T *d, const T *du,
+                           const T *dw, T *x, const int *dims, int ndim, int solvedim) {
+  Lines ln;
+  if (decompose(dims, ndim, solvedim, &ln) != 0) return false;
+  const TuneKey key{(int)sizeof(T), ln.dir, ln.len};
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->tune.count(key)) return false;
+  }
+  const int n =
+      autotune<T>(c, ds, dl, d, du, dw, x, dims, ndim, solvedim, nullptr, 0, 3, nullptr, 0);
+  if (n <= 0) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->tune.emplace(key, Plan{});
+  }
+  return n > 0;
+}
+template bool autotune_on_first_use<float>(Context *, const float *, const float *,
+                                           const float *, const float *, const float *,
+                                           float *, const int *, int, int);
+template bool autotune_on_first_use<double>(Context *, const double *, const double *,
+                                            const double *, const double *, const double *,
+                                            double *, const int *, int, int);
+
 } // namespace pentad
 
 using namespace pentad;

@@ -108,3 +108,40 @@ def test_autotune_keys_by_length_and_precision():
         x32 = torch.from_numpy(ba.astype(np.float32)).cuda()
         P.solve(*t32, x32, dims_a, 0, handle=h)
         assert h.kernel_config(0) == want
+
+
+@pytest.mark.parametrize("dims,sd", [([512, 64, 24], 0), ([32, 384, 16], 1), ([24, 40, 256], 2)])
+def test_env_autotune_on_first_use(dims, sd, monkeypatch):
+    """PENTA_AUTOTUNE=1: the first automatic-choice solve of a (precision,
+    direction, length) tunes on the caller's arrays, solves with the winner
+    and keeps it; without the variable nothing is tuned."""
+    print(f"env autotune dims={dims} sd={sd}", flush=True)
+    coef, b = gen(dims, 70 + sd)
+    ref = O.solve(*coef, b, dims, sd)
+    t = [torch.from_numpy(a).cuda() for a in coef]
+    with P.Handle() as h:
+        cands = {c for c, _ in P.autotune(*t, torch.from_numpy(b.copy()).cuda(), dims, sd, h,
+                                          reps=1)}
+        P.load_library().pentadsolver_clear_tuning(h.value)
+        monkeypatch.delenv("PENTA_AUTOTUNE", raising=False)
+        x = torch.from_numpy(b.copy()).cuda()
+        P.solve(*t, x, dims, sd, handle=h)
+        torch.cuda.synchronize()
+        untuned = h.kernel_config(sd)
+        monkeypatch.setenv("PENTA_AUTOTUNE", "1")
+        cfgs = []
+        for _ in range(2):
+            x = torch.from_numpy(b.copy()).cuda()
+            P.solve(*t, x, dims, sd, handle=h)
+            torch.cuda.synchronize()
+            cfgs.append(h.kernel_config(sd))
+            assert relerr(x.cpu().numpy(), ref) < 1e-12
+        print(f"  untuned {untuned}; tuned {cfgs}; candidates {sorted(cands)}", flush=True)
+        assert cfgs[0] in cands and cfgs[1] == cfgs[0]
+    # a fresh handle with the variable unset keeps the fixed rules
+    monkeypatch.delenv("PENTA_AUTOTUNE", raising=False)
+    with P.Handle() as h:
+        x = torch.from_numpy(b.copy()).cuda()
+        P.solve(*t, x, dims, sd, handle=h)
+        torch.cuda.synchronize()
+        assert h.kernel_config(sd) == untuned
