@@ -146,7 +146,7 @@ pentadStatus_t pentadsolver_S_gpsv_batch_padded(
  * _solve is that assembly followed by the x solve and the y solve of
  * apps/2D_cavity_cpu.cpp:201-219, with the coefficients generated inside the
  * solve kernels: the solution overwrites x (its input values are ignored),
- * no coefficient array is written or read.  nx <= 1024, ny <= 8192 (larger
+ * no coefficient array is written or read.  nx <= 2048, ny <= 8192 (larger
  * grids: _coef + pentadsolver_D_gpsv_batch_padded).  No reference
  * counterpart as one call. */
 pentadStatus_t pentadsolver_D_cavity_pressure_coef(

@@ -52,12 +52,12 @@ __device__ __forceinline__ double cav_c(double ia, double ib, double half_au, do
 // pass each reciprocal to the next point (it is that point's west or north
 // one), three divisions per point instead of five; division is correctly
 // rounded, so the values are the same bit for bit.
-__device__ __forceinline__ void cavity_point_inv(const Cavity &c, int i, int j, double ic,
-                                                 double ie, double iw, double in_, double is_,
-                                                 double &ds, double &dl, double &d, double &du,
-                                                 double &dw, double &rhs) {
+__device__ __forceinline__ void cavity_coef_inv(const Cavity &c, int i, int j, double ic,
+                                                double ie, double iw, double in_, double is_,
+                                                double &ds, double &dl, double &d, double &du,
+                                                double &dw) {
 #pragma clang fp contract(off)
-  const int lu = c.nx + 1, lv = c.nx + 2;
+  const int lv = c.nx + 2;
   const double half_au = 0.5 * c.alpha_u;
   const double dx2 = c.dx * c.dx, dy2 = c.dy * c.dy;
   const double E = cav_c(ic, ie, half_au, dy2);
@@ -88,9 +88,50 @@ __device__ __forceinline__ void cavity_point_inv(const Cavity &c, int i, int j, 
   du = u;
   dw = w;
   d = u + l + s + w;
-  const double p = -(c.up[(int64_t)i * lu + j] - c.up[(int64_t)i * lu + j - 1]);
-  const double q = c.vp[(int64_t)(i - 1) * lv + j] - c.vp[(int64_t)i * lv + j];
-  rhs = __builtin_fma(p, c.dy, -(q * c.dx));
+}
+
+// The right-hand side of point (i, j) from its four face values: u'(i, j),
+// u'(i, j-1), v'(i-1, j), v'(i, j).
+__device__ __forceinline__ double cavity_rhs(const Cavity &c, double ue, double uw, double vn,
+                                             double vs) {
+#pragma clang fp contract(off)
+  const double p = -(ue - uw);
+  const double q = vn - vs;
+  return __builtin_fma(p, c.dy, -(q * c.dx));
+}
+
+__device__ __forceinline__ void cavity_point_inv(const Cavity &c, int i, int j, double ic,
+                                                 double ie, double iw, double in_, double is_,
+                                                 double &ds, double &dl, double &d, double &du,
+                                                 double &dw, double &rhs) {
+  const int lu = c.nx + 1, lv = c.nx + 2;
+  cavity_coef_inv(c, i, j, ic, ie, iw, in_, is_, ds, dl, d, du, dw);
+  rhs = cavity_rhs(c, c.up[(int64_t)i * lu + j], c.up[(int64_t)i * lu + j - 1],
+                   c.vp[(int64_t)(i - 1) * lv + j], c.vp[(int64_t)i * lv + j]);
+}
+
+// v[k] = p[k] for k < N with 16-byte loads.  p is 8-byte aligned; the pairs
+// read are exactly the 16-byte blocks holding p[0] .. p[N-1] (the last load
+// repeats an earlier block when the run starts on a block boundary), so
+// nothing outside the run's blocks is touched.
+template <int N>
+__device__ __forceinline__ void load_run(const double *p, double (&v)[N]) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  const bool odd = (a & 8u) != 0;
+  const double2 *q = reinterpret_cast<const double2 *>(a & ~uintptr_t(15));
+  constexpr int NP = N / 2 + 1;
+  const int last = odd ? N / 2 : (N + 1) / 2 - 1;
+  double2 t[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) t[k] = q[k < last ? k : last];
+  // a select between two loaded values, not between two indices (an index
+  // select would leave the array in scratch memory)
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const double lo = (k & 1) ? t[k >> 1].y : t[k >> 1].x;
+    const double hi = (k & 1) ? t[(k + 1) >> 1].x : t[k >> 1].y;
+    v[k] = odd ? hi : lo;
+  }
 }
 
 __device__ __forceinline__ void cavity_point(const Cavity &c, int i, int j, double &ds,
@@ -136,6 +177,38 @@ __device__ __forceinline__ void cavity_rows(const Cavity &c, int fixed, int r0, 
     const int qc = q <= len + 1 ? q : len + 1;
     return LINE_X ? c.uv[(int64_t)fixed * lv + qc] : c.uv[(int64_t)qc * lv + fixed];
   };
+  if constexpr (LINE_X) {
+    // a whole chunk inside the line: its runs of uv, u' and v' come in with
+    // 16-byte loads (consecutive lanes are M elements apart, so one scalar
+    // load per element would touch a new 128-byte line per lane each time)
+    if (r0 + M <= len) {
+      const int lu = c.nx + 1;
+      const int64_t row = (int64_t)fixed * lv + r0;
+      double a[M + 2], un[M], us[M], ur[M + 1], vn[M], vs[M];
+      load_run(c.uv + row, a);
+      load_run(c.uv + row - lv + 1, un);
+      load_run(c.uv + row + lv + 1, us);
+      load_run(c.up + (int64_t)fixed * lu + r0, ur);
+      load_run(c.vp + row - lv + 1, vn);
+      load_run(c.vp + row + 1, vs);
+      double iw = 1.0 / a[0], ic = 1.0 / a[1];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const int r = r0 + k;
+        const double ie = 1.0 / a[k + 2];
+        cavity_coef_inv(c, fixed, r + 1, ic, ie, iw, 1.0 / un[k], 1.0 / us[k], s[k], l[k],
+                        d[k], u[k], w[k]);
+        iw = ic;
+        ic = ie;
+        f[k] = cavity_rhs(c, ur[k + 1], ur[k], vn[k], vs[k]);
+        if (r < 2) s[k] = 0.0;
+        if (r < 1) l[k] = 0.0;
+        if (r + 1 >= len) u[k] = 0.0;
+        if (r + 2 >= len) w[k] = 0.0;
+      }
+      return;
+    }
+  }
   double iw = 1.0 / along(r0), ic = 1.0 / along(r0 + 1);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
@@ -171,19 +244,28 @@ __device__ __forceinline__ void cavity_rows(const Cavity &c, int fixed, int r0, 
 }
 
 // x solve of every grid row i (system i - 1), L = 2^lshift lanes of M rows
-// each, 256 / L rows per workgroup; the solution goes to x[(i-1)*nx_pad + j-1]
-template <int M>
+// each, 256 / L rows per workgroup; the solution goes to x[(i-1)*nx_pad + j-1].
+// WIDE: more than 64 lanes per row (lines above 64 x M points), the lanes of
+// a row span wavefronts and exchange through LDS; 8-row chunks that way keep
+// three wavefronts per SIMD where 16-row chunks in one wavefront hold one.
+template <int M, bool WIDE>
 __global__ void __launch_bounds__(256) k_cavity_x(Cavity c, double *__restrict__ x,
                                                   int lshift) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = 1 << lshift;
   const int sl = (int)threadIdx.x >> lshift;
   const int p = (int)threadIdx.x & (L - 1);
   const int64_t sys = ((int64_t)blockIdx.x << (8 - lshift)) + sl;
   const int i = (int)(sys < c.ny ? sys : c.ny - 1) + 1;
-  WaveEx ex{(int)(threadIdx.x & 63), p, L};
   double s[M], l[M], d[M], u[M], w[M], f[M];
   cavity_rows<M, true>(c, i, p * M, s, l, d, u, w, f);
-  part_solve<double, M>(s, l, d, u, w, f, p, L, ex);
+  if constexpr (WIDE) {
+    LdsEx ex{smem_raw, sl, p, 256 >> lshift, L};
+    part_solve<double, M>(s, l, d, u, w, f, p, L, ex);
+  } else {
+    WaveEx ex{(int)(threadIdx.x & 63), p, L};
+    part_solve<double, M>(s, l, d, u, w, f, p, L, ex);
+  }
   if (sys < c.ny) {
     double *row = x + (int64_t)(i - 1) * c.nx_pad;
 #pragma unroll
@@ -225,29 +307,40 @@ __global__ void __launch_bounds__(512) k_cavity_y(Cavity c, double *__restrict__
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// 8-row chunks, a power-of-two number of lanes per row: up to 64 in one
+// wavefront, up to 256 (2048 points) across the workgroup
 bool cavity_plan_x(int nx, int *rows, int *lanes) {
-  for (int m : {8, 16}) {
-    int l = 1;
-    while ((int64_t)l * m < nx) l <<= 1;
-    if (l <= 64) {
-      *rows = m;
-      *lanes = l;
-      return true;
-    }
-  }
-  return false;
+  int l = 1;
+  while ((int64_t)l * 8 < nx) l <<= 1;
+  if (l > 256) return false;
+  *rows = 8;
+  *lanes = l;
+  return true;
 }
 
+// The widest workgroup of columns 8-row chunks allow; 16-row chunks (two
+// wavefronts per SIMD instead of three) when 8-row chunks leave fewer than 4
+// columns per workgroup and 16-row ones leave more.  Measured, fused step:
+// 1024 x 2048 0.071 -> 0.054 ms (4 columns of 16-row chunks against 2 of
+// 8-row ones); at 1024 x 1024 4 columns of either are equal (0.036) and 8
+// columns of 16-row chunks slower (0.041).
 bool cavity_plan_y(int ny, int *rows, int *sys_per_wg) {
-  for (int m : {8, 16}) {
-    const int P = (ny + m - 1) / m;
-    for (int S = 64; S >= 1; S >>= 1) {
-      if ((int64_t)S * P <= 512) {
-        *rows = m;
-        *sys_per_wg = S;
-        return true;
-      }
-    }
+  auto widest = [&](int m) {
+    const int64_t P = (ny + m - 1) / m;
+    int S = 64;
+    while (S >= 1 && S * P > 512) S >>= 1;
+    return S; // 0 when not even one column fits
+  };
+  const int s8 = widest(8), s16 = widest(16);
+  if (s8 >= 4 || (s8 >= 1 && s16 <= s8)) {
+    *rows = 8;
+    *sys_per_wg = s8;
+    return true;
+  }
+  if (s16 >= 1) {
+    *rows = 16;
+    *sys_per_wg = s16 < 4 ? s16 : 4;
+    return true;
   }
   return false;
 }
@@ -259,18 +352,27 @@ hipError_t launch_cavity_assemble(const Cavity &c, const CavityOut &o, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_cavity_solve(const Cavity &c, double *x, hipStream_t st) {
-  int mx, lanes, my, S;
-  if (!cavity_plan_x(c.nx, &mx, &lanes) || !cavity_plan_y(c.ny, &my, &S))
+bool cavity_y_supported(int ny, int rows, int sys_per_wg) {
+  if (rows != 8 && rows != 16) return false;
+  if (sys_per_wg < 1 || sys_per_wg > 64 || (sys_per_wg & (sys_per_wg - 1))) return false;
+  return (int64_t)sys_per_wg * ((ny + rows - 1) / rows) <= 512;
+}
+
+hipError_t launch_cavity_solve(const Cavity &c, double *x, int mx, int lanes, int my, int S,
+                               hipStream_t st) {
+  if (mx != 8 || lanes < 1 || lanes > 256 || (lanes & (lanes - 1)) ||
+      (int64_t)lanes * mx < c.nx || !cavity_y_supported(c.ny, my, S))
     return hipErrorInvalidValue;
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
   const int64_t rows_per_wg = 256 >> lshift;
   const int64_t gx = (c.ny + rows_per_wg - 1) / rows_per_wg;
-  if (mx == 8)
-    hipLaunchKernelGGL(k_cavity_x<8>, dim3((unsigned)gx), dim3(256), 0, st, c, x, lshift);
+  if (lanes <= 64)
+    hipLaunchKernelGGL((k_cavity_x<8, false>), dim3((unsigned)gx), dim3(256), 0, st, c, x,
+                       lshift);
   else
-    hipLaunchKernelGGL(k_cavity_x<16>, dim3((unsigned)gx), dim3(256), 0, st, c, x, lshift);
+    hipLaunchKernelGGL((k_cavity_x<8, true>), dim3((unsigned)gx), dim3(256),
+                       (size_t)10 * 256 * sizeof(double), st, c, x, lshift);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int P = (c.ny + my - 1) / my;

@@ -102,15 +102,27 @@ pentadStatus_t pentadsolver_D_cavity_pressure_solve(pentadsolver_handle_t handle
   if (s == PENTAD_STATUS_SUCCESS &&
       (!cavity_plan_x(nx, &mx, &lanes) || !cavity_plan_y(ny, &my, &spw))) {
     std::fprintf(stderr,
-                 "[penta] cavity solve: nx <= 1024 and ny <= 8192 (got %d x %d); assemble with "
+                 "[penta] cavity solve: nx <= 2048 and ny <= 8192 (got %d x %d); assemble with "
                  "pentadsolver_D_cavity_pressure_coef and solve with the padded batch solve\n",
                  nx, ny);
     s = PENTAD_STATUS_INVALID_VALUE;
   }
   if (s == PENTAD_STATUS_SUCCESS) {
+    // pentadsolver_set_kernel_params: y (direction 2 of a 2-D grid) takes
+    // rows per lane and columns per workgroup when they fit
+    int fr, fw;
+    {
+      std::lock_guard<std::mutex> lk(c->mu);
+      fr = c->force_rows[2];
+      fw = c->force_width[2];
+    }
+    if (fr > 0 && fw > 0 && cavity_y_supported(ny, fr, fw)) {
+      my = fr;
+      spw = fw;
+    }
     s = finish(c, launch_cavity_solve(make(nx, ny, nx_pad, alphaU, dx, dy, u_prime, v_prime,
                                            uv),
-                                      x, c->stream),
+                                      x, mx, lanes, my, spw, c->stream),
                "solve");
     if (s == PENTAD_STATUS_SUCCESS) {
       char b0[64], b1[64];

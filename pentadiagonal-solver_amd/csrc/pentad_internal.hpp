@@ -263,7 +263,10 @@ struct CavityOut {
 bool cavity_plan_x(int nx, int *rows, int *lanes);
 bool cavity_plan_y(int ny, int *rows, int *sys_per_wg);
 hipError_t launch_cavity_assemble(const Cavity &c, const CavityOut &o, hipStream_t st);
-// assembly fused into the x solve, then the y solve, in place on x
-hipError_t launch_cavity_solve(const Cavity &c, double *x, hipStream_t st);
+bool cavity_y_supported(int ny, int rows, int sys_per_wg);
+// assembly fused into the x solve (8-row chunks, `lanes` per row), then the
+// y solve (my-row chunks, S columns per workgroup), in place on x
+hipError_t launch_cavity_solve(const Cavity &c, double *x, int mx, int lanes, int my, int S,
+                               hipStream_t st);
 
 } // namespace pentad

@@ -41,7 +41,8 @@ _spec.loader.exec_module(W)
 
 SENT = -3.5
 SHAPES = [(16, 16, 16), (12, 9, 12), (14, 9, 16), (30, 7, 32), (3, 3, 4), (100, 37, 104),
-          (128, 128, 128), (513, 20, 516), (1024, 64, 1024)]
+          (128, 128, 128), (513, 20, 516), (1024, 64, 1024), (1501, 9, 1504),
+          (2048, 6, 2048)]
 XKEYS = ("s_x", "l_x", "d_x", "u_x", "w_x")
 YKEYS = ("s_y", "l_y", "d_y", "u_y", "w_y")
 
@@ -167,10 +168,10 @@ def test_cavity_rejects_bad_arguments():
         P.cavity_pressure_solve(2, 8, 2, 0.7, 0.5, 0.1, up, vp, uv, x)  # nx < 3
     with pytest.raises(P.PentadError):
         P.cavity_pressure_solve(8, 8, 6, 0.7, 0.1, 0.1, up, vp, uv, x)  # nx_pad < nx
-    big = torch.ones(1030 * 8 + 2 * 1030 + 16, dtype=torch.float64, device="cuda")
-    xb = torch.zeros(1030 * 8, dtype=torch.float64, device="cuda")
+    big = torch.ones(2050 * 8 + 2 * 2050 + 16, dtype=torch.float64, device="cuda")
+    xb = torch.zeros(2050 * 8, dtype=torch.float64, device="cuda")
     with pytest.raises(P.PentadError):
-        P.cavity_pressure_solve(1030, 4, 1030, 0.7, 1e-3, 0.25, big, big, big, xb)  # nx > 1024
+        P.cavity_pressure_solve(2050, 4, 2050, 0.7, 1e-3, 0.25, big, big, big, xb)  # nx > 2048
     with pytest.raises(ValueError):
         P.cavity_pressure_solve(64, 64, 64, 0.7, 1e-2, 1e-2, up, vp, uv[:100], x)
 
