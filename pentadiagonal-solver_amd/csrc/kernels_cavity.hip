@@ -16,8 +16,8 @@
 //                      registers and the right-hand side the x solve left
 //                      (S columns x P chunks per workgroup, LDS exchange).
 // The fused pair reads uv, u' and v' (24 B per point) and the field once
-// more for y: 56 B per point for assembly + both solves, against 184 B when
-// the arrays are written and read back.
+// more for y: 56 B per point for assembly + both solves, against 224 B when
+// the eleven arrays are written and the two solves read them back.
 //
 // Arithmetic follows the reference's expressions in their order, with
 // contraction off, except the right-hand side, which the reference's build
