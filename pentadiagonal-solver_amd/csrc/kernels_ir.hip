@@ -102,7 +102,13 @@ __global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
 // more than half a CU's.  SP = false recomputes the separator spikes in every
 // apply instead of keeping them (ir_core.hpp), which the larger chunk needs
 // to stay in registers; SR is the number of stored PCR rounds (IrFactors).
-template <int M, int R, int Q, int BLOCK = 256, bool SP = true, int SR = ir_stored_rounds(R)>
+// IL: the five coefficient arrays interleaved per 4 rows in one array (ds
+// holds [s0..3, l0..3, d0..3, u0..3, w0..3] per group of 4 rows, the other
+// four pointers are unused): one stream of 80-byte groups instead of five
+// streams, the layout the ADI driver writes when it permutes the
+// coefficients.  Needs M % 4 == 0 and line starts at multiples of 4 rows.
+template <int M, int R, int Q, int BLOCK = 256, bool SP = true, int SR = ir_stored_rounds(R),
+          bool IL = false>
 __global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     k_ir_xq(const float *__restrict__ ds, const float *__restrict__ dl,
             const float *__restrict__ d, const float *__restrict__ du,
@@ -126,11 +132,26 @@ __global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
     const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
     float s[M], l[M], dd[M], u[M], w[M];
-    RowsIO<float, M>::load(ds + offc, s);
-    RowsIO<float, M>::load(dl + offc, l);
-    RowsIO<float, M>::load(d + offc, dd);
-    RowsIO<float, M>::load(du + offc, u);
-    RowsIO<float, M>::load(dw + offc, w);
+    if constexpr (IL) {
+      static_assert(M % 4 == 0, "interleaved coefficients come in groups of 4 rows");
+      const float4 *g = reinterpret_cast<const float4 *>(ds) + (offc >> 2) * 5;
+#pragma unroll
+      for (int k = 0; k < M / 4; ++k) {
+        const float4 a0 = g[5 * k], a1 = g[5 * k + 1], a2 = g[5 * k + 2], a3 = g[5 * k + 3],
+                     a4 = g[5 * k + 4];
+        s[4 * k] = a0.x; s[4 * k + 1] = a0.y; s[4 * k + 2] = a0.z; s[4 * k + 3] = a0.w;
+        l[4 * k] = a1.x; l[4 * k + 1] = a1.y; l[4 * k + 2] = a1.z; l[4 * k + 3] = a1.w;
+        dd[4 * k] = a2.x; dd[4 * k + 1] = a2.y; dd[4 * k + 2] = a2.z; dd[4 * k + 3] = a2.w;
+        u[4 * k] = a3.x; u[4 * k + 1] = a3.y; u[4 * k + 2] = a3.z; u[4 * k + 3] = a3.w;
+        w[4 * k] = a4.x; w[4 * k + 1] = a4.y; w[4 * k + 2] = a4.z; w[4 * k + 3] = a4.w;
+      }
+    } else {
+      RowsIO<float, M>::load(ds + offc, s);
+      RowsIO<float, M>::load(dl + offc, l);
+      RowsIO<float, M>::load(d + offc, dd);
+      RowsIO<float, M>::load(du + offc, u);
+      RowsIO<float, M>::load(dw + offc, w);
+    }
     ir_zero_out_of_range<M>(0, s, p == 0, p == L - 1);
     ir_zero_out_of_range<M>(1, l, p == 0, p == L - 1);
     ir_zero_out_of_range<M>(3, u, p == 0, p == L - 1);
@@ -443,6 +464,34 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
   return hipGetLastError();
 }
 
+// the rotated-store kernel over interleaved coefficients (k_ir_xq, IL)
+template <int M, int R>
+hipError_t go_ir_x_il(const float *coef, const double *b, double *out, int64_t n_sys,
+                      int64_t len, const XMap &map, int max_steps, double tol, int *steps,
+                      hipStream_t st) {
+  if constexpr (M % 4 != 0) {
+    return hipErrorInvalidValue;
+  } else {
+    const int block = 256;
+    constexpr int run = (256 >> R) * 8;
+    constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
+    constexpr int SR = ir_stored_rounds(R);
+    const int64_t per_wg = (int64_t)(256 >> R) * Q;
+    const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+    const size_t smem = rotated_tile_bytes<double>(block * Q, M, len);
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(
+          reinterpret_cast<const void *>(&k_ir_xq<M, R, Q, 256, true, SR, true>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_ir_xq<M, R, Q, 256, true, SR, true>), dim3((unsigned)grid),
+                       dim3(block), smem, st, coef, coef, coef, coef, coef, b, out, n_sys, len,
+                       max_steps, tol, steps, map);
+    return hipGetLastError();
+  }
+}
+
 // IR keeps ~ (5 FP32 + 2 FP64 + 6 FP32 factors) per row plus 12 floats per
 // PCR round in registers: workgroups of up to 512 threads (2 waves/SIMD).
 constexpr int IR_YZ_MAXT = 512;
@@ -495,6 +544,27 @@ hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
   IRX(8, 1) IRX(8, 2) IRX(8, 3) IRX(8, 4) IRX(8, 5) IRX(8, 6)
   IRX(10, 1) IRX(10, 2) IRX(10, 3) IRX(10, 4) IRX(10, 5) IRX(10, 6)
 #undef IRX
+  return hipErrorInvalidValue;
+}
+
+bool ir_x_interleave_ok(int rows, int64_t len) {
+  return (rows == 4 || rows == 8) && ir_x_supported(rows, len) && len % 4 == 0;
+}
+
+hipError_t launch_ir_x_map_il(const float *coef, const double *b, double *out, int64_t n_sys,
+                              int64_t len, const XMap &map, int rows, int max_steps, double tol,
+                              int *steps, hipStream_t st) {
+  // every line must start on a group of 4 rows
+  if (!ir_x_interleave_ok(rows, len) || map.c_lo % 4 || map.c_hi % 4 ||
+      (reinterpret_cast<uintptr_t>(coef) & 15u))
+    return hipErrorInvalidValue;
+  const int R = ceil_log2(len / rows);
+#define IRXI(MM, RR) \
+  if (rows == MM && R == RR) \
+    return go_ir_x_il<MM, RR>(coef, b, out, n_sys, len, map, max_steps, tol, steps, st);
+  IRXI(4, 1) IRXI(4, 2) IRXI(4, 3) IRXI(4, 4) IRXI(4, 5) IRXI(4, 6)
+  IRXI(8, 1) IRXI(8, 2) IRXI(8, 3) IRXI(8, 4) IRXI(8, 5) IRXI(8, 6)
+#undef IRXI
   return hipErrorInvalidValue;
 }
 

@@ -120,6 +120,43 @@ hipError_t launch_permute3(const void *in, int in_bytes, void *out, int out_byte
   return hipGetLastError();
 }
 
+// out[20 * g + 4 * k + r] = src[k][4 * g + r]: five arrays interleaved per
+// group of 4 rows, one thread per group (set-up only).
+__global__ void __launch_bounds__(256)
+    k_interleave4(const float4 *__restrict__ s0, const float4 *__restrict__ s1,
+                  const float4 *__restrict__ s2, const float4 *__restrict__ s3,
+                  const float4 *__restrict__ s4, float4 *__restrict__ out, int64_t groups) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    float4 *o = out + 5 * g;
+    o[0] = s0[g];
+    o[1] = s1[g];
+    o[2] = s2[g];
+    o[3] = s3[g];
+    o[4] = s4[g];
+  }
+}
+
+hipError_t launch_interleave4(const float *const src[5], float *out, int64_t n,
+                              hipStream_t st) {
+  if (n % 4) return hipErrorInvalidValue;
+  for (int k = 0; k < 5; ++k)
+    if (reinterpret_cast<uintptr_t>(src[k]) & 15u) return hipErrorInvalidValue;
+  if (reinterpret_cast<uintptr_t>(out) & 15u) return hipErrorInvalidValue;
+  const int64_t groups = n / 4;
+  const int block = 256;
+  int64_t grid = (groups + block - 1) / block;
+  if (grid > 256 * 64) grid = 256 * 64;
+  hipLaunchKernelGGL(k_interleave4, dim3((unsigned)grid), dim3(block), 0, st,
+                     reinterpret_cast<const float4 *>(src[0]),
+                     reinterpret_cast<const float4 *>(src[1]),
+                     reinterpret_cast<const float4 *>(src[2]),
+                     reinterpret_cast<const float4 *>(src[3]),
+                     reinterpret_cast<const float4 *>(src[4]), reinterpret_cast<float4 *>(out),
+                     groups);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_int(int *p, int v) { *p = v; }
 
 // Writes one int with a kernel (usable inside captured graphs, where a

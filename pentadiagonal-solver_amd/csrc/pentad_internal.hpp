@@ -213,6 +213,17 @@ hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
 hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int rows,
                         int sys_per_wg, int max_steps, double tol, int *steps,
                         hipStream_t st);
+// The rotated-store refinement kernel reading the five coefficient arrays
+// interleaved per 4 rows in one array (the layout launch_interleave4 writes):
+// 4- and 8-row chunks, lines of a multiple of 4 rows starting at multiples of
+// 4 elements, coef 16-byte aligned.
+bool ir_x_interleave_ok(int rows, int64_t len);
+hipError_t launch_ir_x_map_il(const float *coef, const double *b, double *out, int64_t n_sys,
+                              int64_t len, const XMap &map, int rows, int max_steps, double tol,
+                              int *steps, hipStream_t st);
+// out[20 * (i / 4) + 4 * k + i % 4] = src[k][i], k < 5, i < n (n % 4 == 0)
+hipError_t launch_interleave4(const float *const src[5], float *out, int64_t n,
+                              hipStream_t st);
 // x direction, any length <= 8 * 64: coefficient spans staged through LDS
 // (rows 8, lanes the next power of two, identity rows past the end)
 bool ir_tr_x_supported(int rows, int64_t len);

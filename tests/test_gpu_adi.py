@@ -134,6 +134,37 @@ def test_adi_multistep_graph_bit_identical_to_eager():
     assert relerr(xe.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize("dims,ratio", [([512, 64, 32], (2.0, 3.0)), ([256, 32, 40], (2.0, 3.0)),
+                                        ([512, 16, 24], (1.05, 1.05)), ([384, 256, 8], (2.0, 3.0))])
+def test_adi_ir_interleaved_coefficients(dims, ratio):
+    """Refinement directions of 4- and 8-row chunks read their coefficients
+    interleaved per 4 rows (one 80-byte group per 4 rows, written when the
+    driver permutes them); the others keep five arrays.  Two steps against
+    the oracle (<= 1e-12, well-conditioned and weak sets), and a captured
+    graph replays to the eager result bit for bit."""
+    coef, b = gen(dims, 31, ratio)
+    print(f"adi ir interleaved dims={dims} ratio={ratio}", flush=True)
+    c32 = [a.astype(np.float32) for a in coef]
+    tc = [torch.from_numpy(a).cuda() for a in c32]
+    ref = oracle_steps([[a.astype(np.float64) for a in c32]], b, dims, 2)
+    xe = torch.from_numpy(b.copy()).cuda()
+    xg = torch.from_numpy(b.copy()).cuda()
+    with P.Adi(P.ADI_IR, dims, [tc]) as adi:
+        adi.step(xe, 2)
+        adi.capture(xg, 2)
+        adi.replay(1)
+        torch.cuda.synchronize()
+        cfg = [adi.config(k) for k in range(len(dims))]
+    err = relerr(xe.cpu().numpy(), ref)
+    print(f"  err={err:.2e} cfg={cfg}", flush=True)
+    for c in cfg:
+        rows = int(c.split("rows=")[1].split()[0])
+        assert ("interleaved" in c) == (rows in (4, 8)), c
+    assert any("interleaved" in c for c in cfg), cfg
+    assert err < 1e-12
+    assert np.array_equal(xg.cpu().numpy(), xe.cpu().numpy())
+
+
 def test_adi_ir_graph_replay_matches_eager():
     dims = [256, 32, 64]
     coef, b = gen(dims, 7)
