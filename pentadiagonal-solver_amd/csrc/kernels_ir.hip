@@ -402,7 +402,7 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
   // passes per workgroup so that a tile row is a whole 128-byte line
   if constexpr (M <= 8) if (rot) {
     constexpr int run = (256 >> R) * 8;
-    constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
+    constexpr int Q = run >= 128 ? 1 : 4;
     const int64_t per_wg = (int64_t)(256 >> R) * Q;
     const int64_t grid = (n_sys + per_wg - 1) / per_wg;
     const size_t smem = rotated_tile_bytes<double>(block * Q, M, len);
@@ -474,7 +474,7 @@ hipError_t go_ir_x_il(const float *coef, const double *b, double *out, int64_t n
   } else {
     const int block = 256;
     constexpr int run = (256 >> R) * 8;
-    constexpr int Q = run >= 128 ? 1 : (run >= 64 ? 2 : 4);
+    constexpr int Q = run >= 128 ? 1 : 4;
     constexpr int SR = ir_stored_rounds(R);
     const int64_t per_wg = (int64_t)(256 >> R) * Q;
     const int64_t grid = (n_sys + per_wg - 1) / per_wg;

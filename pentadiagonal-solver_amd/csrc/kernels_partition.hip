@@ -384,7 +384,10 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
   if constexpr (pipelined) {
     const int run = (256 >> lshift) * (int)sizeof(T);
     if (run < 128) {
-      const int Q = run >= 64 ? 2 : 4;
+      // FP64 takes four passes also where two make a 128-byte run (256^3 ADI
+      // step, 32 lanes per system: 0.604 -> 0.583 ms); FP32 keeps two there
+      // (0.300 vs 0.306 ms with four)
+      const int Q = (run >= 64 && sizeof(T) == 4) ? 2 : 4;
       const int64_t per_wg = (int64_t)(256 >> lshift) * Q;
       const int64_t grid = (n_sys + per_wg - 1) / per_wg;
       const size_t smem = rotated_tile_bytes<T>(256 * Q, M, len);
