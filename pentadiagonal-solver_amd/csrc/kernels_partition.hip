@@ -354,7 +354,10 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
   constexpr bool pipelined = sizeof(T) == 8 ? M <= 8 : M <= 16;
   if (!rot) {
     const int block = 256;
-    if constexpr (pipelined) {
+    // a batch that gives at most two one-pass workgroups per CU runs one
+    // pass per workgroup: passes in sequence only add latency there
+    const bool small = (n_sys * lanes + block - 1) / block <= 512;
+    if constexpr (pipelined) if (!small) {
       // four software-pipelined passes per workgroup (FP64 in place at
       // 256^3: 0.215 -> 0.206 ms against two; equal at 512^3)
       constexpr int Q = 4;
@@ -364,7 +367,7 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                          st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
       return hipGetLastError();
     }
-    if constexpr (sizeof(T) == 8 && (M == 10 || M == 12)) {
+    if constexpr (sizeof(T) == 8 && (M == 10 || M == 12)) if (!small) {
       // two passes, two arrays of the next one loaded ahead (as the rotated
       // path below)
       const int64_t per_wg = (int64_t)(block >> lshift) * 2;
