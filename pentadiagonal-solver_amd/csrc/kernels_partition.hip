@@ -33,11 +33,28 @@ namespace pentad {
 // Q (ROT only): the workgroup solves Q groups of blockDim / L systems one
 // after the other into one tile, so that each row of the tile is a run of
 // Q * blockDim / L systems (wider runs, fewer partial 128-byte lines).
-template <typename T, int M, bool ROT, int Q>
+// M rows of one diagonal stored as C into T (C = float, T = double: the FP32
+// coefficients widened on load, exactly)
+template <typename T, typename C, int M>
+__device__ __forceinline__ void load_coef(const C *__restrict__ src, T (&v)[M]) {
+  if constexpr (sizeof(C) == sizeof(T)) {
+    RowsIO<T, M>::load(src, v);
+  } else {
+    C t[M];
+    RowsIO<C, M>::load(src, t);
+#pragma unroll
+    for (int i = 0; i < M; ++i) v[i] = (T)t[i];
+  }
+}
+
+// C: the coefficients' storage type (float with T = double: one FP64 solve of
+// the FP32 coefficients widened on load, the mixed-precision contract met
+// where no refinement kernel covers the length, see launch_part_x_map_widened)
+template <typename T, int M, bool ROT, int Q, typename C = T>
 __global__ void __launch_bounds__(512)
-    k_part_x(const T *__restrict__ ds, const T *__restrict__ dl,
-             const T *__restrict__ d, const T *__restrict__ du,
-             const T *__restrict__ dw, const T *b, T *out, int64_t n_sys,
+    k_part_x(const C *__restrict__ ds, const C *__restrict__ dl,
+             const C *__restrict__ d, const C *__restrict__ du,
+             const C *__restrict__ dw, const T *b, T *out, int64_t n_sys,
              int64_t len, int lshift /* log2 lanes per system */, XMap map) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t blk = ROT ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
@@ -56,11 +73,11 @@ __global__ void __launch_bounds__(512)
     const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
 
     T s[M], l[M], dd[M], u[M], w[M], f[M];
-    RowsIO<T, M>::load(ds + offc, s);
-    RowsIO<T, M>::load(dl + offc, l);
-    RowsIO<T, M>::load(d + offc, dd);
-    RowsIO<T, M>::load(du + offc, u);
-    RowsIO<T, M>::load(dw + offc, w);
+    load_coef<T, C, M>(ds + offc, s);
+    load_coef<T, C, M>(dl + offc, l);
+    load_coef<T, C, M>(d + offc, dd);
+    load_coef<T, C, M>(du + offc, u);
+    load_coef<T, C, M>(dw + offc, w);
     RowsIO<T, M>::load(b + off, f);
 
     part_solve<T, M>(s, l, dd, u, w, f, p, L, ex);
@@ -91,11 +108,11 @@ __global__ void __launch_bounds__(512)
 // NA: arrays of the next pass loaded ahead (of ds, dl, d, du, dw, b in that
 // order); the rest are loaded after the current pass's solve.  6 = all; fewer
 // where both full sets of rows do not fit the registers.
-template <typename T, int M, int Q, bool ROT, int NA = 6>
+template <typename T, int M, int Q, bool ROT, int NA = 6, typename C = T>
 __global__ void __launch_bounds__(256, 2)
-    k_part_xq(const T *__restrict__ ds, const T *__restrict__ dl,
-              const T *__restrict__ d, const T *__restrict__ du,
-              const T *__restrict__ dw, const T *b, T *out, int64_t n_sys,
+    k_part_xq(const C *__restrict__ ds, const C *__restrict__ dl,
+              const C *__restrict__ d, const C *__restrict__ du,
+              const C *__restrict__ dw, const T *b, T *out, int64_t n_sys,
               int64_t len, int lshift, XMap map) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -116,11 +133,11 @@ __global__ void __launch_bounds__(256, 2)
     const int64_t slo = map.lo(sc), shi = map.hi(sc);
     const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
     const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
-    RowsIO<T, M>::load(ds + offc, s);
-    RowsIO<T, M>::load(dl + offc, l);
-    RowsIO<T, M>::load(d + offc, dd);
-    RowsIO<T, M>::load(du + offc, u);
-    RowsIO<T, M>::load(dw + offc, w);
+    load_coef<T, C, M>(ds + offc, s);
+    load_coef<T, C, M>(dl + offc, l);
+    load_coef<T, C, M>(d + offc, dd);
+    load_coef<T, C, M>(du + offc, u);
+    load_coef<T, C, M>(dw + offc, w);
     RowsIO<T, M>::load(b + off, f);
   };
   // array k of pass q into dst (k: 0..4 the diagonals, 5 the right-hand side)
@@ -132,8 +149,8 @@ __global__ void __launch_bounds__(256, 2)
       RowsIO<T, M>::load(b + slo * map.in_lo + shi * map.in_hi + (int64_t)p * M, dst);
       return;
     }
-    const T *src = k == 0 ? ds : k == 1 ? dl : k == 2 ? d : k == 3 ? du : dw;
-    RowsIO<T, M>::load(src + slo * map.c_lo + shi * map.c_hi + (int64_t)p * M, dst);
+    const C *src = k == 0 ? ds : k == 1 ? dl : k == 2 ? d : k == 3 ? du : dw;
+    load_coef<T, C, M>(src + slo * map.c_lo + shi * map.c_hi + (int64_t)p * M, dst);
   };
   T s[M], l[M], dd[M], u[M], w[M], f[M];
   load(0, s, l, dd, u, w, f);
@@ -346,8 +363,8 @@ __global__ void __launch_bounds__(WIDE ? 512 : 256)
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename T, int M>
-hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
+template <typename T, int M, typename C = T>
+hipError_t go_x(const Diags<C> &a, const T *b, T *out, bool rot, int64_t n_sys,
                 int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
@@ -363,7 +380,7 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
       constexpr int Q = 4;
       const int64_t per_wg = (int64_t)(block >> lshift) * Q;
       const int64_t grid = (n_sys + per_wg - 1) / per_wg;
-      hipLaunchKernelGGL((k_part_xq<T, M, Q, false>), dim3((unsigned)grid), dim3(block), 0,
+      hipLaunchKernelGGL((k_part_xq<T, M, Q, false, 6, C>), dim3((unsigned)grid), dim3(block), 0,
                          st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
       return hipGetLastError();
     }
@@ -372,12 +389,12 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
       // path below)
       const int64_t per_wg = (int64_t)(block >> lshift) * 2;
       const int64_t grid = (n_sys + per_wg - 1) / per_wg;
-      hipLaunchKernelGGL((k_part_xq<T, M, 2, false, 2>), dim3((unsigned)grid), dim3(block), 0,
+      hipLaunchKernelGGL((k_part_xq<T, M, 2, false, 2, C>), dim3((unsigned)grid), dim3(block), 0,
                          st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
       return hipGetLastError();
     }
     const int64_t grid = (n_sys * lanes + block - 1) / block;
-    hipLaunchKernelGGL((k_part_x<T, M, false, 1>), dim3((unsigned)grid), dim3(block), 0,
+    hipLaunchKernelGGL((k_part_x<T, M, false, 1, C>), dim3((unsigned)grid), dim3(block), 0,
                        st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
     return hipGetLastError();
   }
@@ -406,7 +423,7 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                            a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
         return hipGetLastError();
       };
-      return Q == 2 ? go(k_part_xq<T, M, 2, true>) : go(k_part_xq<T, M, 4, true>);
+      return Q == 2 ? go(k_part_xq<T, M, 2, true, 6, C>) : go(k_part_xq<T, M, 4, true, 6, C>);
     }
   }
   // FP64 chunks of 10 / 12 rows: both full sets of rows do not fit, so the
@@ -431,7 +448,7 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                            a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
         return hipGetLastError();
       };
-      return Q == 2 ? go(k_part_xq<T, M, 2, true, 2>) : go(k_part_xq<T, M, 4, true, 2>);
+      return Q == 2 ? go(k_part_xq<T, M, 2, true, 2, C>) : go(k_part_xq<T, M, 4, true, 2, C>);
     }
   }
   // one pass, or chunks too large to hold two sets of rows: FP32 in
@@ -453,9 +470,9 @@ hipError_t go_x(const Diags<T> &a, const T *b, T *out, bool rot, int64_t n_sys,
                        a.du, a.dw, b, out, n_sys, len, lshift, map);
     return hipGetLastError();
   };
-  if (Q == 1) return go(k_part_x<T, M, true, 1>);
-  if (Q == 2) return go(k_part_x<T, M, true, 2>);
-  return go(k_part_x<T, M, true, 4>);
+  if (Q == 1) return go(k_part_x<T, M, true, 1, C>);
+  if (Q == 2) return go(k_part_x<T, M, true, 2, C>);
+  return go(k_part_x<T, M, true, 4, C>);
 }
 
 // Largest workgroup each (type, M) instantiation is compiled for.  With W
@@ -512,6 +529,21 @@ hipError_t launch_part_x_map(const Diags<T> &a, const T *b, T *out, bool rot,
   case 10: return go_x<T, 10>(a, b, out, rot, n_sys, len, map, lanes, st);
   case 12: return go_x<T, 12>(a, b, out, rot, n_sys, len, map, lanes, st);
   case 16: return go_x<T, 16>(a, b, out, rot, n_sys, len, map, lanes, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_part_x_map_widened(const Diags<float> &a, const double *b, double *out,
+                                     bool rot, int64_t n_sys, int64_t len, const XMap &map,
+                                     int rows, int lanes, hipStream_t st) {
+  if (!part_x_supported(8, rows, lanes) || rows * lanes != len) return hipErrorInvalidValue;
+  switch (rows) {
+  case 4: return go_x<double, 4, float>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 6: return go_x<double, 6, float>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 8: return go_x<double, 8, float>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 10: return go_x<double, 10, float>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 12: return go_x<double, 12, float>(a, b, out, rot, n_sys, len, map, lanes, st);
+  case 16: return go_x<double, 16, float>(a, b, out, rot, n_sys, len, map, lanes, st);
   }
   return hipErrorInvalidValue;
 }

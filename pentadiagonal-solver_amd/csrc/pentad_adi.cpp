@@ -209,6 +209,12 @@ hipError_t solve_dir(Adi *A, int d, const void *in, void *out, bool first_step,
   if (A->mode == 4)
     return launch_sf_x_map<float>((const float *)A->tab[d], (const float *)in, (float *)out,
                                   true, nsys, len, map, pl.rows, st);
+  if (A->mode == 2 && pl.kind == Plan::IrWideX)
+    return launch_part_x_map_widened(
+        Diags<float>{(const float *)A->coef[d][0], (const float *)A->coef[d][1],
+                     (const float *)A->coef[d][2], (const float *)A->coef[d][3],
+                     (const float *)A->coef[d][4]},
+        (const double *)in, (double *)out, true, nsys, len, map, pl.rows, pl.lanes, st);
   if (A->mode == 2 && A->il[d])
     return launch_ir_x_map_il((const float *)A->coef[d][0], (const double *)in, (double *)out,
                               nsys, len, map, pl.rows, A->max_refine, A->tol, A->d_steps, st);
@@ -372,7 +378,11 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
       continue;
     }
     A->plan[d] = (mode == 2) ? plan_ir(c, ln, aligned) : plan_solve(c, ln, A->cbytes, aligned, false);
-    if (A->plan[d].kind != Plan::PartX && A->plan[d].kind != Plan::IrX) A->rotating = false;
+    // refinement mode: lengths no refinement kernel covers run the FP64
+    // solve of the widened coefficients with the same rotated stores
+    if (A->plan[d].kind != Plan::PartX && A->plan[d].kind != Plan::IrX &&
+        A->plan[d].kind != Plan::IrWideX)
+      A->rotating = false;
   }
   // the caller's x must be 16-byte aligned for the vector loads of the first
   // x-solve; that is checked when stepping

@@ -438,7 +438,28 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
         return pl;
       }
     }
-    staged(pl);
+    if (staged(pl)) return pl;
+    // lengths no refinement kernel covers (past 64 chunks of 10 rows, or
+    // no power-of-two split): one FP64 solve of the coefficients widened on
+    // load -- the system the refinement converges to, in one pass over the
+    // same 36 B per row -- with the FP64 x kernel of whole chunks, or for
+    // lines past 64 x 16 rows the workgroup-wide kernel (stride 1).  These
+    // lengths took the one-lane kernel with its scratch passes before
+    // (768-row lines: 12 vs 1.7 ms for the FP64 solve of the same step).
+    for (int r : {8, 6, 10, 12, 16, 4}) {
+      const int64_t lanes = ln.len / r;
+      if (ln.len % r == 0 && part_x_supported(8, r, (int)lanes)) {
+        pl.kind = Plan::IrWideX;
+        pl.rows = r;
+        pl.lanes = (int)lanes;
+        return pl;
+      }
+    }
+    Plan q;
+    if (plan_x_long(ln, 8, q)) {
+      q.kind = Plan::IrWideYZ;
+      return q;
+    }
     return pl;
   }
   if (frows > 0 && ir_yz_supported(frows, fwidth, ln.len)) {
@@ -533,6 +554,9 @@ std::string plan_config(const Plan &pl) {
     std::snprintf(buf, sizeof buf, "ir widened rows=%d sys_per_wg=%d", pl.rows,
                   pl.sys_per_wg);
     return buf;
+  case Plan::IrWideX:
+    std::snprintf(buf, sizeof buf, "ir widened rows=%d lanes=%d", pl.rows, pl.lanes);
+    return buf;
   case Plan::SfX:
     std::snprintf(buf, sizeof buf, "shared-fact rows=%d lanes=%d", pl.rows, pl.lanes);
     return buf;
@@ -599,6 +623,9 @@ hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
     return launch_ir_tr_x(a, x, ln, pl.rows, max_steps, tol, steps, st);
   case Plan::IrWideYZ: // no refinement sweep: *steps is left as it is
     return launch_part_yz_widened(a, x, ln, pl.rows, pl.sys_per_wg, st);
+  case Plan::IrWideX: // the same along x, in place
+    return launch_part_x_map_widened(a, x, x, false, ln.n_sys, ln.len, xmap_inplace(ln),
+                                     pl.rows, pl.lanes, st);
   case Plan::IrLane:
     if (buffer == nullptr) return hipErrorInvalidValue;
     return launch_ir_lane(a, x, ln, buffer, max_steps, tol, steps, st);

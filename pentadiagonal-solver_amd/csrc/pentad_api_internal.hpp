@@ -15,7 +15,7 @@ namespace pentad {
 struct Plan {
   enum Kind {
     Naive = 0, PartX = 1, PartYZ = 2, IrLane = 3, IrX = 4, IrYZ = 5, SfX = 6, SfYZ = 7,
-    TrX = 8, IrTrX = 9, IrWideYZ = 10
+    TrX = 8, IrTrX = 9, IrWideYZ = 10, IrWideX = 11
   } kind = Naive;
   int rows = 0;       // rows per lane (partitioned kernels)
   int lanes = 0;      // lanes per system (x)

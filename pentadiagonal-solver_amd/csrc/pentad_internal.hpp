@@ -217,6 +217,13 @@ hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int r
 // interleaved per 4 rows in one array (the layout launch_interleave4 writes):
 // 4- and 8-row chunks, lines of a multiple of 4 rows starting at multiples of
 // 4 elements, coef 16-byte aligned.
+// The x kernels over FP32 coefficients widened to FP64 on load, FP64 b and
+// out: one FP64 solve of the system the refinement converges to, for the
+// lengths no refinement kernel covers (the rows / lanes of
+// launch_part_x_map<double>).
+hipError_t launch_part_x_map_widened(const Diags<float> &a, const double *b, double *out,
+                                     bool rot, int64_t n_sys, int64_t len, const XMap &map,
+                                     int rows, int lanes, hipStream_t st);
 bool ir_x_interleave_ok(int rows, int64_t len);
 hipError_t launch_ir_x_map_il(const float *coef, const double *b, double *out, int64_t n_sys,
                               int64_t len, const XMap &map, int rows, int max_steps, double tol,

@@ -165,6 +165,35 @@ def test_adi_ir_interleaved_coefficients(dims, ratio):
     assert np.array_equal(xg.cpu().numpy(), xe.cpu().numpy())
 
 
+@pytest.mark.parametrize("dims", [[768, 24, 16], [16, 1024, 12], [24, 16, 768]])
+def test_adi_ir_long_lines_widened(dims):
+    """Refinement-mode directions whose length no refinement kernel covers
+    (768 = 12 x 64, 1024 = 16 x 64 rows) keep the rotating layouts with the
+    FP64 solve of the widened coefficients ("ir widened"): two steps against
+    the oracle of the widened coefficients (<= 1e-12), graph replay equal to
+    the eager steps bit for bit."""
+    coef, b = gen(dims, 41)
+    print(f"adi ir long dims={dims}", flush=True)
+    c32 = [a.astype(np.float32) for a in coef]
+    tc = [torch.from_numpy(a).cuda() for a in c32]
+    ref = oracle_steps([[a.astype(np.float64) for a in c32]], b, dims, 2)
+    xe = torch.from_numpy(b.copy()).cuda()
+    xg = torch.from_numpy(b.copy()).cuda()
+    with P.Adi(P.ADI_IR, dims, [tc]) as adi:
+        adi.step(xe, 2)
+        adi.capture(xg, 2)
+        adi.replay(1)
+        torch.cuda.synchronize()
+        cfg = [adi.config(k) for k in range(3)]
+    err = relerr(xe.cpu().numpy(), ref)
+    print(f"  err={err:.2e} cfg={cfg}", flush=True)
+    for k, n in enumerate(dims):
+        if n >= 768:
+            assert cfg[k].startswith("ir widened rows="), cfg
+    assert err < 1e-12
+    assert np.array_equal(xg.cpu().numpy(), xe.cpu().numpy())
+
+
 def test_adi_ir_graph_replay_matches_eager():
     dims = [256, 32, 64]
     coef, b = gen(dims, 7)

@@ -201,3 +201,29 @@ def test_ir_x_forced_transpose_on_exact_length():
         x, _ = run_ir(a32, b, dims, 0, handle=h)
         assert h.kernel_config(0) == "ir transpose rows=8 lanes=64"
     assert relerr(x, reference(a32, b, dims, 0)) < TOL
+
+
+@pytest.mark.parametrize("n,cfg", [(768, "ir widened rows=12 lanes=64"),
+                                   (1024, "ir widened rows=16 lanes=64"),
+                                   (640 + 128, "ir widened rows=12 lanes=64"),
+                                   (2048, "ir widened rows="), (1000, "ir widened rows="),
+                                   (5000, "ir widened rows=")])
+@pytest.mark.parametrize("cond", ["well", "weak"])
+def test_ir_x_long_lines_widened(n, cfg, cond):
+    """x lengths no refinement kernel covers (past 64 chunks of 10 rows, or
+    no power-of-two split past 512 rows): one FP64 solve of the coefficients
+    widened on load, by the FP64 x kernel of whole chunks or, past 64 x 16
+    rows and for ragged lengths, the workgroup-wide kernel -- the system the
+    refinement converges to (<= 1e-12 against the oracle of the widened
+    coefficients, no refinement step), where these lengths took the
+    one-lane kernel with its scratch passes before."""
+    dims = [n, 7, 3]
+    print(f"ir x long len={n} {cond}", flush=True)
+    a32, b = gen32(dims, n + 11, (2.0, 3.0) if cond == "well" else (1.05, 1.05))
+    with P.Handle() as h:
+        x, steps = run_ir(a32, b, dims, 0, handle=h)
+        got = h.kernel_config(0)
+    print(f"  {got} err={relerr(x, reference(a32, b, dims, 0)):.2e}", flush=True)
+    assert got.startswith(cfg), got
+    assert relerr(x, reference(a32, b, dims, 0)) < TOL
+    assert steps == 0
