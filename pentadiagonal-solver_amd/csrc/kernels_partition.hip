@@ -47,22 +47,14 @@ __device__ __forceinline__ void load_coef(const C *__restrict__ src, T (&v)[M]) 
   }
 }
 
-// C: the coefficients' storage type (float with T = double: one FP64 solve of
-// the FP32 coefficients widened on load, the mixed-precision contract met
-// where no refinement kernel covers the length, see launch_part_x_map_widened)
-template <typename T, int M, bool ROT, int Q, typename C = T>
-__global__ void __launch_bounds__(512)
-    k_part_x(const C *__restrict__ ds, const C *__restrict__ dl,
-             const C *__restrict__ d, const C *__restrict__ du,
-             const C *__restrict__ dw, const T *b, T *out, int64_t n_sys,
-             int64_t len, int lshift /* log2 lanes per system */, XMap map) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int64_t blk = ROT ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-  const int L = 1 << lshift;
-  const int s0shift = __builtin_ctz(blockDim.x) - lshift; // systems per pass
-  const int sl = (int)threadIdx.x >> lshift;
-  const int p = (int)threadIdx.x & (L - 1);
-  WaveEx ex{(int)(threadIdx.x & 63), p, L};
+// the Q passes of k_part_x (and its tile write-out) over exchange Ex
+template <typename T, int M, bool ROT, int Q, typename C, typename Ex>
+__device__ __forceinline__ void part_x_passes(const C *__restrict__ ds, const C *__restrict__ dl,
+                                              const C *__restrict__ d, const C *__restrict__ du,
+                                              const C *__restrict__ dw, const T *b, T *out,
+                                              int64_t n_sys, int64_t len, const XMap &map,
+                                              int64_t blk, int L, int s0shift, int sl, int p,
+                                              Ex &ex, unsigned char *smem_raw) {
 #pragma unroll 1
   for (int q = 0; q < Q; ++q) {
     const int64_t sys = ((blk * Q + q) << s0shift) + sl;
@@ -95,32 +87,50 @@ __global__ void __launch_bounds__(512)
   }
 }
 
-// Rotated store in Q > 1 passes, software-pipelined: the rows of pass q + 1
-// are loaded before pass q is solved, so the loads of one pass overlap the
-// arithmetic of the previous one.  The 64 KB tile allows two workgroups of
-// 256 threads per CU (2 waves per SIMD), which leaves up to 256 VGPRs for the
-// two sets of rows (FP64 M <= 8, FP32 M <= 16; larger chunks keep k_part_x).
-// Measured at 512^3: FP64 1.56 -> 1.44 ms, FP32 (4 passes of 8 systems
-// instead of 2 passes of 16 in 512-thread workgroups) 0.87 -> 0.73 ms.
-// ROT = false: the same pipelined passes solving in place (out == b, no
-// tile; measured at 512^3: FP64 1.55 -> 1.44 ms with Q = 2, FP32 0.81 ->
-// 0.76 ms with Q = 4).
-// NA: arrays of the next pass loaded ahead (of ds, dl, d, du, dw, b in that
-// order); the rest are loaded after the current pass's solve.  6 = all; fewer
-// where both full sets of rows do not fit the registers.
-template <typename T, int M, int Q, bool ROT, int NA = 6, typename C = T>
-__global__ void __launch_bounds__(256, 2)
-    k_part_xq(const C *__restrict__ ds, const C *__restrict__ dl,
-              const C *__restrict__ d, const C *__restrict__ du,
-              const C *__restrict__ dw, const T *b, T *out, int64_t n_sys,
-              int64_t len, int lshift, XMap map) {
+// C: the coefficients' storage type (float with T = double: one FP64 solve of
+// the FP32 coefficients widened on load, the mixed-precision contract met
+// where no refinement kernel covers the length, see launch_part_x_map_widened)
+// WIDE: more than 64 lanes per system (128 or 256): the lanes of a system
+// span wavefronts and exchange through LDS (LdsEx, 10 values per thread),
+// placed after the rotated tile.  Long FP64 lines that way keep 8-row chunks
+// (three wavefronts per SIMD) where 16-row chunks in one wavefront hold two.
+template <typename T, int M, bool ROT, int Q, typename C = T, bool WIDE = false>
+__global__ void __launch_bounds__(512)
+    k_part_x(const C *__restrict__ ds, const C *__restrict__ dl,
+             const C *__restrict__ d, const C *__restrict__ du,
+             const C *__restrict__ dw, const T *b, T *out, int64_t n_sys,
+             int64_t len, int lshift /* log2 lanes per system */, XMap map) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t blk = ROT ? xcd_tile(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int L = 1 << lshift;
+  const int s0shift = __builtin_ctz(blockDim.x) - lshift; // systems per pass
+  const int sl = (int)threadIdx.x >> lshift;
+  const int p = (int)threadIdx.x & (L - 1);
+  if constexpr (WIDE) {
+    const size_t tb = ROT ? rotated_tile_bytes<T>((int)blockDim.x * Q, M, len) : 0;
+    LdsEx ex{smem_raw + ((tb + 15) & ~(size_t)15), sl, p, (int)blockDim.x >> lshift, L};
+    part_x_passes<T, M, ROT, Q, C>(ds, dl, d, du, dw, b, out, n_sys, len, map, blk, L,
+                                   s0shift, sl, p, ex, smem_raw);
+  } else {
+    WaveEx ex{(int)(threadIdx.x & 63), p, L};
+    part_x_passes<T, M, ROT, Q, C>(ds, dl, d, du, dw, b, out, n_sys, len, map, blk, L,
+                                   s0shift, sl, p, ex, smem_raw);
+  }
+}
+
+// the passes of k_part_xq over exchange Ex
+template <typename T, int M, int Q, bool ROT, int NA, typename C, typename Ex>
+__device__ __forceinline__ void part_xq_passes(const C *__restrict__ ds, const C *__restrict__ dl,
+                                               const C *__restrict__ d, const C *__restrict__ du,
+                                               const C *__restrict__ dw, const T *b, T *out,
+                                               int64_t n_sys, int64_t len, int lshift,
+                                               const XMap &map, Ex &ex,
+                                               unsigned char *smem_raw) {
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
   const int L = 1 << lshift;
   const int s0shift = 8 - lshift; // systems per pass (256 threads)
   const int sl = (int)threadIdx.x >> lshift;
   const int p = (int)threadIdx.x & (L - 1);
-  WaveEx ex{(int)(threadIdx.x & 63), p, L};
   auto offset = [&](int q) {
     const int64_t sys = ((blk * Q + q) << s0shift) + sl;
     const int64_t sc = sys < n_sys ? sys : n_sys - 1;
@@ -201,10 +211,46 @@ __global__ void __launch_bounds__(256, 2)
   }
   if constexpr (ROT) {
     __syncthreads();
-    constexpr int qshift = Q == 4 ? 2 : (Q == 2 ? 1 : 0);
-    const int sshift = s0shift + qshift;
+    const int sshift = s0shift + __builtin_ctz(Q);
     tile_flush<T, M>(reinterpret_cast<const T *>(smem_raw), sshift, (int)len, out, n_sys,
                      blk << sshift, map);
+  }
+}
+
+// Rotated store in Q > 1 passes, software-pipelined: the rows of pass q + 1
+// are loaded before pass q is solved, so the loads of one pass overlap the
+// arithmetic of the previous one.  The 64 KB tile allows two workgroups of
+// 256 threads per CU (2 waves per SIMD), which leaves up to 256 VGPRs for the
+// two sets of rows (FP64 M <= 8, FP32 M <= 16; larger chunks keep k_part_x).
+// Measured at 512^3: FP64 1.56 -> 1.44 ms, FP32 (4 passes of 8 systems
+// instead of 2 passes of 16 in 512-thread workgroups) 0.87 -> 0.73 ms.
+// ROT = false: the same pipelined passes solving in place (out == b, no
+// tile; measured at 512^3: FP64 1.55 -> 1.44 ms with Q = 2, FP32 0.81 ->
+// 0.76 ms with Q = 4).
+// NA: arrays of the next pass loaded ahead (of ds, dl, d, du, dw, b in that
+// order); the rest are loaded after the current pass's solve.  6 = all; fewer
+// where both full sets of rows do not fit the registers.
+// WIDE: 128 or 256 lanes per system, exchanged through LDS after the tile
+// (as k_part_x).
+template <typename T, int M, int Q, bool ROT, int NA = 6, typename C = T, bool WIDE = false>
+__global__ void __launch_bounds__(256, 2)
+    k_part_xq(const C *__restrict__ ds, const C *__restrict__ dl,
+              const C *__restrict__ d, const C *__restrict__ du,
+              const C *__restrict__ dw, const T *b, T *out, int64_t n_sys,
+              int64_t len, int lshift, XMap map) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int L = 1 << lshift;
+  const int sl = (int)threadIdx.x >> lshift;
+  const int p = (int)threadIdx.x & (L - 1);
+  if constexpr (WIDE) {
+    const size_t tb = ROT ? rotated_tile_bytes<T>(256 * Q, M, len) : 0;
+    LdsEx ex{smem_raw + ((tb + 15) & ~(size_t)15), sl, p, 256 >> lshift, L};
+    part_xq_passes<T, M, Q, ROT, NA, C>(ds, dl, d, du, dw, b, out, n_sys, len, lshift, map,
+                                        ex, smem_raw);
+  } else {
+    WaveEx ex{(int)(threadIdx.x & 63), p, L};
+    part_xq_passes<T, M, Q, ROT, NA, C>(ds, dl, d, du, dw, b, out, n_sys, len, lshift, map,
+                                        ex, smem_raw);
   }
 }
 
@@ -361,6 +407,12 @@ __global__ void __launch_bounds__(WIDE ? 512 : 256)
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// chunk sizes of the wide x kernels (more than 64 lanes per system): 8 rows
+// for FP64 (three wavefronts per SIMD), 8 or 16 for FP32
+template <typename T> constexpr bool wide_rows(int M) {
+  return M == 8 || (sizeof(T) == 4 && M == 16);
+}
+
 namespace {
 
 template <typename T, int M, typename C = T>
@@ -368,6 +420,54 @@ hipError_t go_x(const Diags<C> &a, const T *b, T *out, bool rot, int64_t n_sys,
                 int64_t len, const XMap &map, int lanes, hipStream_t st) {
   int lshift = 0;
   while ((1 << lshift) < lanes) ++lshift;
+  if (lanes > 64) {
+    // lanes of a system across wavefronts (part_x_wide_supported):
+    // 256-thread workgroups, the exchange after the rotated tile
+    if constexpr (wide_rows<T>(M)) {
+      constexpr int block = 256;
+      const size_t xb = (size_t)10 * block * sizeof(T);
+      auto go = [&](auto kernel, int64_t grid, size_t smem) {
+        if (smem > 65536) {
+          const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)smem);
+          if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), smem, st, a.ds, a.dl,
+                           a.d, a.du, a.dw, b, out, n_sys, len, lshift, map);
+        return hipGetLastError();
+      };
+      const int sys_per_pass = block >> lshift;
+      if (!rot) {
+        // in place: two software-pipelined passes per workgroup; one pass
+        // where that leaves at most two workgroups per CU (FP64 1024-row
+        // lines, 32768 systems: 0.40 -> 0.39 ms, 128 systems: 7.8 against
+        // 12.9 us with two passes; four passes 0.46 ms)
+        if ((n_sys + sys_per_pass - 1) / sys_per_pass <= 512)
+          return go(k_part_x<T, M, false, 1, C, true>,
+                    (n_sys + sys_per_pass - 1) / sys_per_pass, xb);
+        const int64_t per_wg = (int64_t)sys_per_pass * 2;
+        return go(k_part_xq<T, M, 2, false, 6, C, true>, (n_sys + per_wg - 1) / per_wg, xb);
+      }
+      // rotated: passes per workgroup for the widest tile row up to a
+      // 128-byte run that fits the 160 KB of LDS with the exchange, software-
+      // pipelined (FP64 1024-row lines, 8 passes: 0.57 -> 0.50 ms)
+      auto bytes = [&](int q) {
+        return ((rotated_tile_bytes<T>(block * q, M, len) + 15) & ~(size_t)15) + xb;
+      };
+      int Q = 1;
+      while (Q < 8 && sys_per_pass * Q * (int)sizeof(T) < 128 && bytes(2 * Q) <= 163840) Q *= 2;
+      const int64_t per_wg = (int64_t)sys_per_pass * Q;
+      const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+      switch (Q) {
+      case 1: return go(k_part_x<T, M, true, 1, C, true>, grid, bytes(1));
+      case 2: return go(k_part_xq<T, M, 2, true, 6, C, true>, grid, bytes(2));
+      case 4: return go(k_part_xq<T, M, 4, true, 6, C, true>, grid, bytes(4));
+      default: return go(k_part_xq<T, M, 8, true, 6, C, true>, grid, bytes(8));
+      }
+    }
+    return hipErrorInvalidValue;
+  }
   constexpr bool pipelined = sizeof(T) == 8 ? M <= 8 : M <= 16;
   if (!rot) {
     const int block = 256;
@@ -516,11 +616,17 @@ bool part_x_supported(int elem_bytes, int rows, int lanes) {
          rows == 16;
 }
 
+bool part_x_wide_supported(int elem_bytes, int rows, int lanes) {
+  if (lanes != 128 && lanes != 256) return false;
+  return elem_bytes == 8 ? wide_rows<double>(rows) : wide_rows<float>(rows);
+}
+
 template <typename T>
 hipError_t launch_part_x_map(const Diags<T> &a, const T *b, T *out, bool rot,
                              int64_t n_sys, int64_t len, const XMap &map, int rows,
                              int lanes, hipStream_t st) {
-  if (!part_x_supported(sizeof(T), rows, lanes) || rows * lanes != len)
+  if (!(part_x_supported(sizeof(T), rows, lanes) || part_x_wide_supported(sizeof(T), rows, lanes)) ||
+      rows * lanes != len)
     return hipErrorInvalidValue;
   switch (rows) {
   case 4: return go_x<T, 4>(a, b, out, rot, n_sys, len, map, lanes, st);
@@ -536,7 +642,9 @@ hipError_t launch_part_x_map(const Diags<T> &a, const T *b, T *out, bool rot,
 hipError_t launch_part_x_map_widened(const Diags<float> &a, const double *b, double *out,
                                      bool rot, int64_t n_sys, int64_t len, const XMap &map,
                                      int rows, int lanes, hipStream_t st) {
-  if (!part_x_supported(8, rows, lanes) || rows * lanes != len) return hipErrorInvalidValue;
+  if (!(part_x_supported(8, rows, lanes) || part_x_wide_supported(8, rows, lanes)) ||
+      rows * lanes != len)
+    return hipErrorInvalidValue;
   switch (rows) {
   case 4: return go_x<double, 4, float>(a, b, out, rot, n_sys, len, map, lanes, st);
   case 6: return go_x<double, 6, float>(a, b, out, rot, n_sys, len, map, lanes, st);

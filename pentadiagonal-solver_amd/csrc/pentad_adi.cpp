@@ -377,7 +377,7 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
       }
       continue;
     }
-    A->plan[d] = (mode == 2) ? plan_ir(c, ln, aligned) : plan_solve(c, ln, A->cbytes, aligned, false);
+    A->plan[d] = (mode == 2) ? plan_ir(c, ln, aligned) : plan_solve(c, ln, A->cbytes, aligned);
     // refinement mode: lengths no refinement kernel covers run the FP64
     // solve of the widened coefficients with the same rotated stores
     if (A->plan[d].kind != Plan::PartX && A->plan[d].kind != Plan::IrX &&

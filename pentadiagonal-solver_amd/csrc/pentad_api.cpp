@@ -164,8 +164,7 @@ bool plan_x_long(const Lines &ln, int elem_bytes, Plan &pl) {
   return false;
 }
 
-Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
-                const void *const ptrs[6], bool in_place) {
+Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const ptrs[6]) {
   Plan pl;
   std::string want = requested_algo(c, ln.dir);
   // x: an unrecognised name runs the automatic choice, as the reference's
@@ -255,7 +254,8 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
     const bool al = x_aligned(ln, elem_bytes, ptrs);
     if (al && (want == "auto" || want == "thomas-pcr")) {
       if (have_tuned && tuned.kind == Plan::PartX &&
-          part_x_supported(elem_bytes, tuned.rows, tuned.lanes) &&
+          (part_x_supported(elem_bytes, tuned.rows, tuned.lanes) ||
+           part_x_wide_supported(elem_bytes, tuned.rows, tuned.lanes)) &&
           (int64_t)tuned.rows * tuned.lanes == ln.len) {
         pl.kind = Plan::PartX;
         pl.rows = tuned.rows;
@@ -279,13 +279,23 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
       static const int pref32[] = {16, 12, 8, 10, 6, 4};
       const int *pref = is64 ? pref64 : pref32;
       const int npref = 6;
+      // lines of 128 or 256 chunks of 8 rows (FP64) / 16 rows (FP32): the
+      // wide kernel, a system's lanes across wavefronts (in place, 2^25
+      // rows: FP64 1024-row lines 0.39 ms against 1.08 on the LDS-staged
+      // kernel and 1.22 with 16 x 64; 2048 rows 0.37 against 1.03; FP32
+      // 2048 / 4096 rows 0.21 against 0.50)
+      {
+        const int r = is64 ? 8 : 16;
+        const int64_t lanes = ln.len / r;
+        if (ln.len % r == 0 && lanes <= 256 && part_x_wide_supported(elem_bytes, r, (int)lanes)) {
+          pl.kind = Plan::PartX;
+          pl.rows = r;
+          pl.lanes = (int)lanes;
+          return pl;
+        }
+      }
       for (int k = 0; k < npref; ++k) {
         const int r = pref[k];
-        // FP64 in place, 16 rows x 64 lanes (1024 rows): the LDS-staged
-        // kernel is faster (0.22 against 0.31 ms for 2^23 rows, 1.72 against
-        // 2.42 ms for 1024 x 256 x 256); "thomas-pcr" by name keeps it
-        if (is64 && r == 16 && ln.len == 1024 && in_place && ln.dense() && want == "auto")
-          continue;
         if (ln.len % r) continue;
         const int64_t lanes = ln.len / r;
         if (lanes > 64 || lanes < 2 || (lanes & (lanes - 1))) continue;
@@ -446,6 +456,13 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
     // lines past 64 x 16 rows the workgroup-wide kernel (stride 1).  These
     // lengths took the one-lane kernel with its scratch passes before
     // (768-row lines: 12 vs 1.7 ms for the FP64 solve of the same step).
+    // Lines of 128 or 256 chunks of 8 rows take the wide x kernel first.
+    if (ln.len % 8 == 0 && part_x_wide_supported(8, 8, (int)(ln.len / 8))) {
+      pl.kind = Plan::IrWideX;
+      pl.rows = 8;
+      pl.lanes = (int)(ln.len / 8);
+      return pl;
+    }
     for (int r : {8, 6, 10, 12, 16, 4}) {
       const int64_t lanes = ln.len / r;
       if (ln.len % r == 0 && part_x_supported(8, r, (int)lanes)) {

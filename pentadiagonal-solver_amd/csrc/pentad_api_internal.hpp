@@ -59,10 +59,7 @@ std::string requested_algo(Context *c, int dir);
 std::string env_algo(int dir);
 bool debug_launch();
 
-// in_place = false: the caller (the ADI driver's rotated layouts) needs the
-// partitioned x kernel whenever one fits, never the LDS-staged one
-Plan plan_solve(Context *c, const Lines &ln, int elem_bytes,
-                const void *const ptrs[6], bool in_place = true);
+Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const ptrs[6]);
 Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]);
 size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes);
 template <typename T>

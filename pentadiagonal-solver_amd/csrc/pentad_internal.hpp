@@ -75,6 +75,10 @@ template <typename T>
 hipError_t launch_part_x(const Diags<T> &a, T *x, const Lines &ln, int rows,
                          int lanes, hipStream_t st);
 bool part_x_supported(int elem_bytes, int rows, int lanes);
+// The same solve with 128 or 256 lanes per system (lines of 1024 / 2048 rows
+// in 8-row chunks, FP32 also 16-row): the lanes of a system span wavefronts
+// and exchange through LDS.  launch_part_x / launch_part_x_map accept these.
+bool part_x_wide_supported(int elem_bytes, int rows, int lanes);
 // Addressing of a batch of contiguous lines for the x-like kernels (used by
 // the ADI driver, whose field and coefficient layouts differ per direction).
 // System s splits as s_lo = s % S1, s_hi = s / S1 (64-bit):

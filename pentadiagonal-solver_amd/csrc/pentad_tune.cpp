@@ -57,7 +57,8 @@ std::vector<Plan> candidates(const Lines &ln, int elem_bytes, bool ir,
       if (!aligned) continue;
       const int64_t lanes = ln.len / r;
       const bool ok = ir ? ir_x_supported(r, ln.len)
-                         : (lanes <= 64 && part_x_supported(elem_bytes, r, (int)lanes));
+                         : ((lanes <= 64 && part_x_supported(elem_bytes, r, (int)lanes)) ||
+                            part_x_wide_supported(elem_bytes, r, (int)lanes));
       if (ok) {
         Plan p;
         p.kind = ir ? Plan::IrX : Plan::PartX;

@@ -194,6 +194,47 @@ def test_adi_ir_long_lines_widened(dims):
     assert np.array_equal(xg.cpu().numpy(), xe.cpu().numpy())
 
 
+@pytest.mark.parametrize("dims,mode", [([1024, 20], "fp64"), ([20, 1024], "fp64"),
+                                       ([1024, 24, 16], "fp64"), ([24, 16, 1024], "fp64"),
+                                       ([2048, 12], "fp64"), ([2048, 20], "fp32"),
+                                       ([24, 16, 2048], "fp32"), ([1024, 24, 16], "ir"),
+                                       ([2048, 12], "ir")])
+def test_adi_wide_lines(dims, mode):
+    """Directions of 128 or 256 chunks (1024 / 2048 rows) on the wide x
+    kernel with the rotated store: several systems per pass (8-pass tiles,
+    the last workgroup partly empty in the 2-D cases) or one; two steps
+    against the oracle, graph replay equal to the eager steps bit for bit."""
+    coef, b = gen(dims, 43)
+    print(f"adi wide dims={dims} mode={mode}", flush=True)
+    if mode == "fp64":
+        tc = [torch.from_numpy(a).cuda() for a in coef]
+        x0 = b.copy()
+        ref = oracle_steps([coef], b, dims, 2)
+        tol, m = 1e-12, P.ADI_FP64
+    else:
+        c32 = [a.astype(np.float32) for a in coef]
+        tc = [torch.from_numpy(a).cuda() for a in c32]
+        x0 = b.astype(np.float32) if mode == "fp32" else b.copy()
+        ref = oracle_steps([[a.astype(np.float64) for a in c32]], x0.astype(np.float64), dims, 2)
+        tol, m = (5e-5, P.ADI_FP32) if mode == "fp32" else (1e-12, P.ADI_IR)
+    xe = torch.from_numpy(x0.copy()).cuda()
+    xg = torch.from_numpy(x0.copy()).cuda()
+    with P.Adi(m, dims, [tc]) as adi:
+        adi.step(xe, 2)
+        adi.capture(xg, 2)
+        adi.replay(1)
+        torch.cuda.synchronize()
+        cfg = [adi.config(k) for k in range(len(dims))]
+    err = relerr(xe.double().cpu().numpy(), ref)
+    print(f"  err={err:.2e} cfg={cfg}", flush=True)
+    for k, n in enumerate(dims):
+        if n >= 1024:
+            r = 16 if mode == "fp32" else 8
+            assert f"rows={r} lanes={n // r}" in cfg[k], cfg
+    assert err < tol
+    assert np.array_equal(xg.cpu().numpy(), xe.cpu().numpy())
+
+
 def test_adi_ir_graph_replay_matches_eager():
     dims = [256, 32, 64]
     coef, b = gen(dims, 7)

@@ -204,16 +204,17 @@ def test_ir_x_forced_transpose_on_exact_length():
 
 
 @pytest.mark.parametrize("n,cfg", [(768, "ir widened rows=12 lanes=64"),
-                                   (1024, "ir widened rows=16 lanes=64"),
+                                   (1024, "ir widened rows=8 lanes=128"),
                                    (640 + 128, "ir widened rows=12 lanes=64"),
-                                   (2048, "ir widened rows="), (1000, "ir widened rows="),
+                                   (2048, "ir widened rows=8 lanes=256"),
+                                   (1000, "ir widened rows="),
                                    (5000, "ir widened rows=")])
 @pytest.mark.parametrize("cond", ["well", "weak"])
 def test_ir_x_long_lines_widened(n, cfg, cond):
     """x lengths no refinement kernel covers (past 64 chunks of 10 rows, or
     no power-of-two split past 512 rows): one FP64 solve of the coefficients
-    widened on load, by the FP64 x kernel of whole chunks or, past 64 x 16
-    rows and for ragged lengths, the workgroup-wide kernel -- the system the
+    widened on load, by the FP64 x kernel of whole chunks (up to 256 lanes of
+    8 rows) or, for other lengths, the workgroup-wide kernel -- the system the
     refinement converges to (<= 1e-12 against the oracle of the widened
     coefficients, no refinement step), where these lengths took the
     one-lane kernel with its scratch passes before."""

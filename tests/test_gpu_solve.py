@@ -310,20 +310,73 @@ def test_long_x_lines_on_chip(n, dtype):
     than 64 lanes per system (separators exchanged through LDS across the
     workgroup), not the one-lane kernel with its scratch passes; the same
     lines through the strided partitioned kernel at stride 1 (what a padded
-    layout whose pitch breaks 16-byte alignment runs) via "thomas-pcr"."""
+    layout whose pitch breaks 16-byte alignment runs) via "thomas-pcr".
+    Lengths of 128 or 256 whole chunks (4096 FP32 rows) take the wide
+    partitioned kernel both ways."""
     dims = [n, 9, 3]
     print(f"long x len={n} {np.dtype(dtype).name}", flush=True)
+    r = 8 if dtype == np.float64 else 16
+    wide = n % r == 0 and n // r in (128, 256)
     with P.Handle() as h:
         check(gen(dims, n, dtype), dims, 0, h)
         cfg = h.kernel_config(0)
-        assert P.kernel_that_ran(0) == "transpose"
         print(f"  {cfg}", flush=True)
-        assert cfg.startswith("transpose rows=") and int(cfg.split("lanes=")[1]) > 64, cfg
+        if wide:  # whole chunks: the wide partitioned kernel instead
+            assert cfg == f"thomas-pcr rows={r} lanes={n // r}", cfg
+        else:
+            assert P.kernel_that_ran(0) == "transpose"
+            assert cfg.startswith("transpose rows=") and int(cfg.split("lanes=")[1]) > 64, cfg
         h.set_algo(0, "thomas-pcr")
         check(gen(dims, n + 1, dtype), dims, 0, h)
         cfg = h.kernel_config(0)
         print(f"  {cfg}", flush=True)
-        assert cfg.startswith("thomas-pcr rows=") and "sys_per_wg" in cfg, cfg
+        if wide:
+            assert cfg == f"thomas-pcr rows={r} lanes={n // r}", cfg
+        else:
+            assert cfg.startswith("thomas-pcr rows=") and "sys_per_wg" in cfg, cfg
+
+
+@pytest.mark.parametrize("dims,dtype,forced", [
+    ([1024, 9, 3], np.float64, None),      # 27 systems, one pass per workgroup
+    ([1024, 467, 3], np.float64, None),    # 1401 systems, two pipelined passes
+    ([2048, 1031], np.float64, None),      # one system per pass, ragged grid
+    ([2048, 5, 3], np.float32, None),
+    ([2048, 1027], np.float32, None),      # two pipelined passes, ragged
+    ([4096, 3, 7], np.float32, None),
+    ([1024, 600], np.float32, (8, 128)),   # FP32 8-row chunks by parameters
+    ([2048, 40], np.float32, (8, 256)),
+    ([1024, 1000], np.float64, (8, 128)),
+])
+def test_wide_x_lines(dims, dtype, forced):
+    """x lines of 128 or 256 chunks: a system's lanes span wavefronts and
+    exchange the reduced system through LDS.  Against the oracle at the
+    FP64 / FP32 bars; batches that leave the last workgroup partly empty
+    (clamped systems must not write), one- and two-pass launches."""
+    n = dims[0]
+    print(f"wide x dims={dims} {np.dtype(dtype).name} forced={forced}", flush=True)
+    with P.Handle() as h:
+        if forced:
+            h.set_kernel_params(0, *forced)
+        err = check(gen(dims, n + len(dims), dtype), dims, 0, h)
+        cfg = h.kernel_config(0)
+        print(f"  {cfg} err={err:.2e}", flush=True)
+        r, lanes = forced or ((8 if dtype == np.float64 else 16), n // (8 if dtype == np.float64 else 16))
+        assert cfg == f"thomas-pcr rows={r} lanes={lanes}", cfg
+
+
+def test_wide_x_matches_transpose_bitwise():
+    """FP64: the wide kernel and the LDS-staged kernel run the same
+    partition (8-row chunks, 128 lanes), so their results agree bit for
+    bit."""
+    dims = [1024, 33, 5]
+    arrs = gen(dims, 77)
+    with P.Handle() as h:
+        a = gpu_solve(arrs, dims, 0, h)
+        assert h.kernel_config(0) == "thomas-pcr rows=8 lanes=128"
+        h.set_algo(0, "transpose")
+        b = gpu_solve(arrs, dims, 0, h)
+        assert h.kernel_config(0) == "transpose rows=8 lanes=128"
+    assert np.array_equal(a, b)
 
 
 def test_long_x_lines_full_batch():
