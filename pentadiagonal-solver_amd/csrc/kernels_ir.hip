@@ -107,8 +107,10 @@ __global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
 // four pointers are unused): one stream of 80-byte groups instead of five
 // streams, the layout the ADI driver writes when it permutes the
 // coefficients.  Needs M % 4 == 0 and line starts at multiples of 4 rows.
+// X16: the 16-system tile without pads (tile_put_x16), for tiles that
+// would not fit two workgroups per CU with them.
 template <int M, int R, int Q, int BLOCK = 256, bool SP = true, int SR = ir_stored_rounds(R),
-          bool IL = false>
+          bool IL = false, bool X16 = false>
 __global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     k_ir_xq(const float *__restrict__ ds, const float *__restrict__ dl,
             const float *__restrict__ d, const float *__restrict__ du,
@@ -161,13 +163,22 @@ __global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     double xs[M];
     const int k = ir_solve<M, SR, SP>(A, b, xs, p, L, ex, max_steps, tol);
     kmax = k > kmax ? k : kmax;
-    tile_put<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, Q << s0shift,
-                        (q << s0shift) + sl);
+    if constexpr (X16) {
+      static_assert((Q << s0shift) == 16, "X16 tiles hold 16 systems");
+      tile_put_x16<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, (q << s0shift) + sl);
+    } else {
+      tile_put<double, M>(reinterpret_cast<double *>(smem_raw), xs, p, Q << s0shift,
+                          (q << s0shift) + sl);
+    }
   }
   __syncthreads();
   constexpr int sshift = s0shift + (Q == 4 ? 2 : (Q == 2 ? 1 : 0));
-  tile_flush<double, M>(reinterpret_cast<const double *>(smem_raw), sshift, (int)len, out,
-                        n_sys, blk << sshift, map);
+  if constexpr (X16)
+    tile_flush_x16<double, M>(reinterpret_cast<const double *>(smem_raw), (int)len, out, n_sys,
+                              blk << sshift, map);
+  else
+    tile_flush<double, M>(reinterpret_cast<const double *>(smem_raw), sshift, (int)len, out,
+                          n_sys, blk << sshift, map);
   if (lane == 0) note_steps(steps_out, kmax);
 }
 
@@ -416,9 +427,28 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
                        a.dl, a.d, a.du, a.dw, b, out, n_sys, len, max_steps, tol, steps, map);
     return hipGetLastError();
   }
-  // 10 rows per lane: 512-thread workgroups (a tile of 16 systems of 640
-  // rows is 82 KB), spikes recomputed, two stored PCR rounds (255 VGPRs, no
-  // spill; three stored rounds or kept spikes spill)
+  // 10 rows per lane: spikes recomputed, two stored PCR rounds (255 VGPRs,
+  // no spill; three stored rounds or kept spikes spill).  640-row lines (64
+  // lanes): 256-thread workgroups, four passes into an unpadded 16-system
+  // tile (80 KB; 82 KB with the pads, too much for two per CU), two
+  // workgroups per CU as for 8-row chunks (384x256x640 hyperdiffusion
+  // direction: 0.453 -> 0.436 ms against the 512-thread kernel below, one
+  // workgroup per CU; the same results bit for bit)
+  if constexpr (M == 10 && R == 6) if (rot) {
+    constexpr int Q = 4;
+    const int64_t per_wg = 16;
+    const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+    const size_t smem = sizeof(double) * 16 * (size_t)len;
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_ir_xq<M, R, Q, 256, false, 2, false, true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ir_xq<M, R, Q, 256, false, 2, false, true>), dim3((unsigned)grid),
+                       dim3(block), smem, st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len,
+                       max_steps, tol, steps, map);
+    return hipGetLastError();
+  }
+  // other 10-row lines: 512-thread workgroups
   if constexpr (M == 10) if (rot) {
     constexpr int B = 512;
     constexpr int run = (B >> R) * 8;

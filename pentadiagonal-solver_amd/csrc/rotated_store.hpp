@@ -74,6 +74,31 @@ __device__ __forceinline__ void tile_flush(const T *tile, int sshift, int len, T
   }
 }
 
+// Tiles of exactly 16 systems without the per-chunk pad: chunk p's column
+// col is stored at col ^ (p & 15).  A wavefront's puts (one column, 64
+// chunks) then fall on 16 bank pairs instead of all on one, and the rows the
+// flush reads stay whole 128-byte runs; the pad's LDS is saved, which is
+// what lets two workgroups' tiles of 640-row lines (80 KB each) share one
+// CU's 160 KB.
+template <typename T, int M>
+__device__ __forceinline__ void tile_put_x16(T *tile, const T (&v)[M], int p, int col) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) tile[(p * M + i) * 16 + (col ^ (p & 15))] = v[i];
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void tile_flush_x16(const T *tile, int len, T *out, int64_t n_sys,
+                                               int64_t sys0, const XMap &map) {
+  const int s = (int)threadIdx.x & 15;
+  const int64_t sys = sys0 + s;
+  if (sys >= n_sys) return;
+  T *o = out + map.lo(sys) * map.out_lo + map.hi(sys) * map.out_hi;
+  const int rstep = (int)blockDim.x >> 4;
+  for (int row = (int)threadIdx.x >> 4; row < len; row += rstep)
+    __builtin_nontemporal_store(tile[row * 16 + (s ^ ((row / M) & 15))],
+                                o + (int64_t)row * map.out_row);
+}
+
 template <typename T, int M>
 __device__ __forceinline__ void stage_rotated_store(T *tile, const T (&v)[M], int p,
                                                     int lshift, int len, T *out,

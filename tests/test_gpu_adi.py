@@ -341,3 +341,28 @@ def test_adi_c5_line_lengths(dims, mode, cond):
     if mode == "ir":
         assert 1 <= steps <= 5, steps
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("dims", [[640, 12], [20, 640], [640, 24]])
+def test_adi_ir_640_rows_partial_tile(dims):
+    """Refinement ADI steps whose 640-row direction has a system count that
+    is not a multiple of the 16-system tile (the unpadded-tile kernel's last
+    workgroup holds clamped systems it must not write), against the oracle
+    of the widened FP32 coefficients: max-norm relative error <= 1e-12.
+    (The rotating driver needs every length to split into a power of two of
+    4- to 10-row chunks, so 3-D grids always give multiples of 16 systems;
+    2-D grids do not.)"""
+    coef, b = gen(dims, 41)
+    print(f"adi ir 640 partial dims={dims}", flush=True)
+    c32 = [a.astype(np.float32) for a in coef]
+    tc = [torch.from_numpy(a).cuda() for a in c32]
+    x = torch.from_numpy(b.copy()).cuda()
+    with P.Adi(P.ADI_IR, dims, [tc]) as adi:
+        adi.step(x, 2)
+        torch.cuda.synchronize()
+        got = x.cpu().numpy()
+        cfg = [adi.config(k) for k in range(len(dims))]
+    err = relerr(got, oracle_steps([[a.astype(np.float64) for a in c32]], b, dims, 2))
+    print(f"  err={err:.2e} cfg={cfg}", flush=True)
+    assert cfg[dims.index(640)] == "ir thomas-pcr rows=10 lanes=64", cfg
+    assert err < 1e-12
