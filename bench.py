@@ -862,6 +862,25 @@ def main():
                 best = min(best, e0.elapsed_time(e1))
             probe[f"{streams}_streams"] = {"ms": best, "GBs": 36 * n / (best * 1e-3) / 1e9}
             del cf
+        # the same bytes with the headline kernel's per-lane load shape:
+        # coalesced writes at full occupancy, and the kernel's
+        # workgroup structure with its rotated tile store, with 0..150 rounds
+        # of 8 FP64 FMAs per pass standing in for the solve (the kernel
+        # itself issues about 850 VALU instructions per pass)
+        cf = torch.zeros(5 * n, dtype=torch.float32, device="cuda")
+        layouts = [(0, 0)] + ([(1, r) for r in (0, 25, 50, 100, 150)] if dims == [512] * 3 else [])
+        for layout, rounds in layouts:
+            best = 1e30
+            for _ in range(10):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(torch.cuda.current_stream())
+                P.probe_ir_layout(layout, rounds, cf, b, xo, handle=h)
+                e1.record(torch.cuda.current_stream())
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            key = "lanes_coalesced_writes" if layout == 0 else f"kernel_structure_rotated_valu{8 * rounds}"
+            probe[key] = {"ms": best, "GBs": 36 * n / (best * 1e-3) / 1e9}
+        del cf
         del xo
         log(f"bench: refinement byte-mix probe {probe}")
     log("bench: drop-in in-place FP64 direction solves")

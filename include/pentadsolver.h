@@ -290,6 +290,21 @@ pentadStatus_t pentadsolver_probe_ir_mix(pentadsolver_handle_t handle, int strea
                                          const float *const coef[5], const double *b,
                                          double *x, size_t n);
 
+/* The FP32 + FP64 refinement kernel's I/O without its solve, per 8-row chunk
+ * (one per thread): 10 float4 of coefficients interleaved per 4 rows (coef:
+ * 5n floats, the ADI driver's layout) and 4 double2 of b read at the lane
+ * strides of the solve kernel, then `rounds` rounds of 8 FP64 FMAs standing
+ * in for the solve's arithmetic.  layout = 0: each wavefront's 512 rows of
+ * x written with coalesced 16-byte stores, one chunk per thread at full
+ * occupancy (n a multiple of 2048); layout = 1: the headline kernel's workgroup structure (256 threads,
+ * two workgroups per CU, four passes of four 512-row systems into a
+ * 16-system LDS tile written rotated as 128-byte runs 4 KB apart), n =
+ * 512^3 exactly.  No reference counterpart (a variant of
+ * apps/roofline_probe_cuda.cu:27-49). */
+pentadStatus_t pentadsolver_probe_ir_layout(pentadsolver_handle_t handle, int layout,
+                                            int rounds, const float *coef, const double *b,
+                                            double *x, size_t n);
+
 /* ADI sweep driver.  One step solves along every direction in turn (x, then
  * y, then z for ndim = 3) in place on a field x in the standard layout
  * (t_dims[0] fastest), the loop of the reference's apps/adi_cuda.cu:167-176.

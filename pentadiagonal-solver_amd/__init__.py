@@ -67,6 +67,7 @@ _SIGNATURES = {
     "pentadsolver_clear_tuning": (None, [_P]),
     "pentadsolver_probe_stream": (_I, [_P, ctypes.POINTER(_P), _P, _SZ, _I, _I]),
     "pentadsolver_probe_ir_mix": (_I, [_P, _I, ctypes.POINTER(_P), _P, _P, _SZ]),
+    "pentadsolver_probe_ir_layout": (_I, [_P, _I, _I, _P, _P, _P, _SZ]),
     "pentadsolver_adi_create": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), _I, _I, _I, ctypes.c_double]),
     "pentadsolver_adi_create_stencil": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "pentadsolver_D_gpsv_stencil_batch": (_I, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_double, _P, _IP, _I, _I]),
@@ -421,6 +422,16 @@ def probe_ir_mix(streams, coef, b, x, handle=None):
     arr = (_P * 5)(*ptrs)
     _check(load_library().pentadsolver_probe_ir_mix(_hval(handle), int(streams), arr, _ptr(b),
                                                     _ptr(x), x.numel()))
+
+
+def probe_ir_layout(layout, rounds, coef, b, x, handle=None):
+    """The refinement kernel's I/O without its solve: layout 0 (the kernel's
+    per-lane loads, coalesced writes in line order) or 1 (the headline
+    kernel's workgroup structure and rotated tile store, 512^3 rows only);
+    rounds x 8 FP64 FMAs per thread stand in for the solve.  coef: one
+    float32 tensor of 5n floats (interleaved per 4 rows)."""
+    _check(load_library().pentadsolver_probe_ir_layout(
+        _hval(handle), int(layout), int(rounds), _ptr(coef), _ptr(b), _ptr(x), x.numel()))
 
 
 ADI_FP64, ADI_FP32, ADI_IR, ADI_SHARED_FP64, ADI_SHARED_FP32 = 0, 1, 2, 3, 4

@@ -111,4 +111,95 @@ hipError_t launch_probe_ir_mix(int streams, const float *const coef[5], const do
   return hipGetLastError();
 }
 
+// The refinement kernel's I/O without its solve, in two layouts.  Rows come
+// in 8-row chunks, one per thread: 10 float4 of coefficients interleaved per
+// 4 rows (lane stride 160 B, the driver's layout) and 4 double2 of b (lane
+// stride 64 B), x = b + some of the coefficients, then `rounds` rounds of
+// 8 dependent-free FP64 FMAs on the chunk's values stand in for the solve's
+// arithmetic (0 = none).
+//   LANES: one chunk per thread, every thread of a 256-thread workgroup on
+//          its own chunk; x written over the wavefront's 512 rows with
+//          coalesced 16-byte stores (1 KB per instruction, the values'
+//          order within those rows is immaterial here); as many workgroups
+//          as the rows need.
+//   TILE:  the headline kernel's workgroup structure for 512-row lines:
+//          256 threads, two workgroups per CU (66 KB of LDS each), four
+//          passes of four systems (one per wavefront) into one 16-system
+//          tile, written rotated after a barrier (a 128-byte run per tile
+//          row, rows 4 KB apart, non-temporal stores), the x direction of a
+//          512^3 field (system s: lo = s % 512, hi = s / 512; row r of s at
+//          lo + 512 r + 512^2 hi).
+template <bool TILE>
+__global__ void __launch_bounds__(256, 2)
+    k_probe_ir_layout(const float4 *__restrict__ c, const double2 *__restrict__ b,
+                      double *__restrict__ x, int rounds) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *tile = reinterpret_cast<double *>(smem_raw);
+  const int w = (int)threadIdx.x >> 6, l = (int)threadIdx.x & 63;
+  const int passes = TILE ? 4 : 1;
+  for (int q = 0; q < passes; ++q) {
+    const size_t chunk = TILE ? ((size_t)blockIdx.x * 16 + q * 4 + w) * 64 + l
+                              : (size_t)blockIdx.x * 256 + threadIdx.x;
+    float4 a[10];
+    double2 f[4];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a[k] = c[chunk * 10 + k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) f[j] = b[chunk * 4 + j];
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[2 * j] = f[j].x + (double)(a[j].x + a[j + 5].y + a[(j + 2) % 10].z);
+      v[2 * j + 1] = f[j].y + (double)(a[j + 1].w + a[j + 4].x + a[(j + 7) % 10].y);
+    }
+    for (int it = 0; it < rounds; ++it) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = fma(v[i], 0.9990000000001, 1e-3);
+    }
+    if constexpr (TILE) {
+      const int col = q * 4 + w;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) tile[(l * 8 + i) * 16 + col + l] = v[i];
+    } else {
+      // the wavefront's 512 rows as four 1 KB stores, lanes side by side
+      // (streaming stores, as the solve kernels write whole lines)
+      double *o = x + ((chunk - l) * 4 + l) * 2;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        __builtin_nontemporal_store(v[2 * j], o + j * 128);
+        __builtin_nontemporal_store(v[2 * j + 1], o + j * 128 + 1);
+      }
+    }
+  }
+  if constexpr (TILE) {
+    __syncthreads();
+    const int s = (int)threadIdx.x & 15;
+    const size_t sys = (size_t)blockIdx.x * 16 + s;
+    double *o = x + (sys & 511) + (sys >> 9) * (512 * 512);
+    for (int row = (int)threadIdx.x >> 4; row < 512; row += 16)
+      __builtin_nontemporal_store(tile[row * 16 + s + row / 8], o + (size_t)row * 512);
+  }
+}
+
+hipError_t launch_probe_ir_layout(int layout, int rounds, const float *coef, const double *b,
+                                  double *x, size_t n, hipStream_t st) {
+  const float4 *c = reinterpret_cast<const float4 *>(coef);
+  const double2 *b2 = reinterpret_cast<const double2 *>(b);
+  if (layout == 0) {
+    if (n % 2048) return hipErrorInvalidValue; // whole workgroups of 8-row chunks
+    hipLaunchKernelGGL(k_probe_ir_layout<false>, dim3((unsigned)(n / 2048)), dim3(256), 0, st,
+                       c, b2, x, rounds);
+  } else {
+    if (n != (size_t)512 * 512 * 512) return hipErrorInvalidValue;
+    const size_t smem = sizeof(double) * (16 * 512 + 64);
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_probe_ir_layout<true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_probe_ir_layout<true>, dim3((unsigned)(n / (16 * 512))), dim3(256),
+                       smem, st, c, b2, x, rounds);
+  }
+  return hipGetLastError();
+}
+
 } // namespace pentad

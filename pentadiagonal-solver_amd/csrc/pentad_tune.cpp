@@ -26,6 +26,8 @@ hipError_t launch_probe(const void *const in[6], void *out, size_t n,
                         int elem_bytes, int fmas, hipStream_t st);
 hipError_t launch_probe_ir_mix(int streams, const float *const coef[5], const double *b,
                                double *x, size_t n, hipStream_t st);
+hipError_t launch_probe_ir_layout(int layout, int rounds, const float *coef, const double *b,
+                                  double *x, size_t n, hipStream_t st);
 
 namespace {
 
@@ -349,6 +351,17 @@ pentadStatus_t pentadsolver_probe_ir_mix(pentadsolver_handle_t h, int streams,
   for (int i = 0; i < need; ++i)
     if (!coef[i]) return PENTAD_STATUS_INVALID_VALUE;
   const hipError_t e = launch_probe_ir_mix(streams, coef, b, x, n, ctx_of(h)->stream);
+  return e == hipSuccess ? PENTAD_STATUS_SUCCESS : PENTAD_STATUS_EXECUTION_FAILED;
+}
+
+pentadStatus_t pentadsolver_probe_ir_layout(pentadsolver_handle_t h, int layout, int rounds,
+                                            const float *coef, const double *b, double *x,
+                                            size_t n) {
+  if (!coef || !b || !x || n == 0 || rounds < 0 || (layout != 0 && layout != 1))
+    return PENTAD_STATUS_INVALID_VALUE;
+  if (layout == 0 ? (n % 2048 != 0) : (n != (size_t)512 * 512 * 512))
+    return PENTAD_STATUS_INVALID_VALUE;
+  const hipError_t e = launch_probe_ir_layout(layout, rounds, coef, b, x, n, ctx_of(h)->stream);
   return e == hipSuccess ? PENTAD_STATUS_SUCCESS : PENTAD_STATUS_EXECUTION_FAILED;
 }
 
