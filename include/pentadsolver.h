@@ -422,7 +422,9 @@ const char *pentadsolver_kernel_config(pentadsolver_handle_t handle, int dir);
  * system k owning elements offsets[k] .. offsets[k+1]-1 of every array
  * (offsets: host array of n_sys + 1 entries; lengths 3 .. 8192).  create
  * builds a device schedule once (systems sorted longest first and grouped
- * into work items whose lanes per system follow the length); each solve is
+ * into work items whose lanes per system follow the length; the longest
+ * quarter of the items is claimed alternating with the shortest, the rest
+ * longest first); each solve is
  * then one persistent launch on the handle's stream with no allocation and
  * no host synchronisation (capturable in a hipGraph).  The workgroups take
  * work items from a counter held in the plan, so two solves of one plan

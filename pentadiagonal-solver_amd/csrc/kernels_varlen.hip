@@ -10,9 +10,11 @@
 // length are virtual identity rows (d = 1, no couplings, zero right-hand
 // side), which leaves the real rows' solution unchanged; couplings of real
 // rows past the end are zeroed.  The workgroups are persistent and take the
-// items of the longest-first list in order from a device counter (reset by
-// the launcher): longest-processing-time-first scheduling, so the long
-// items start first and the short ones fill in behind them.  (Dealing item
+// items in list order from a device counter (reset by the launcher).  The
+// host lists the longest quarter of the items alternating with the
+// shortest, then the rest longest first (pentad_varlen.cpp, claim_order):
+// long items start first and short ones fill in behind them, while the
+// mixed sizes keep the CUs from loading and solving in step.  (Dealing item
 // b + kG to workgroup b instead left a tail: 1035 single-system items on
 // 256 one-workgroup CUs took 0.21 ms, 1024 took 0.15 ms.)
 //
