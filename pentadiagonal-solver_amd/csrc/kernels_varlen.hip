@@ -34,6 +34,7 @@ struct VarEx {
   unsigned char *raw; // K * N elements
   int t, p, P, N;
   bool synced = false; // the last workgroup-wide step was all() (as LdsEx)
+  float *red = nullptr; // N / 64 floats of LDS apart from raw (max_f32), or none
   template <int K, typename T>
   __device__ __forceinline__ void swapc(const T *send, T *from_prev, T *from_next,
                                         int h) {
@@ -56,7 +57,22 @@ struct VarEx {
     synced = true;
     return r;
   }
+  // With red and systems of whole wavefronts (P >= 64): the maximum of each
+  // wavefront by DPP, one slot per wavefront in LDS, and every thread takes
+  // the maximum of its system's P / 64 slots -- two barriers, where the
+  // exchange rounds below take two per round (18 for 512 chunks).
   __device__ __forceinline__ float max_f32(float v) {
+    if (red != nullptr && P >= 64) {
+      v = WaveEx{t & 63, t & 63, 64}.max_f32(v);
+      if (!synced) __syncthreads(); // the slots' previous readers are done
+      if ((t & 63) == 0) red[t >> 6] = v;
+      __syncthreads();
+      const int w0 = (t - p) >> 6; // the system's first wavefront
+      float m = red[w0];
+      for (int k = 1; k < (P >> 6); ++k) m = fmaxf(m, red[w0 + k]);
+      synced = false;
+      return m;
+    }
     for (int h = 1; h < P; h <<= 1) {
       float a[1] = {v}, bp[1], bn[1];
       swapc<1>(a, bp, bn, h);
@@ -250,6 +266,7 @@ __global__ void __launch_bounds__(VL_BLOCK)
       k = ir_solve<M, 0, false>(A, b, xs, p, P, ex, max_steps, tol);
     } else {
       VarEx ex{smem_raw, t, p, P, VL_BLOCK};
+      ex.red = reinterpret_cast<float *>(smem_raw + (size_t)10 * VL_BLOCK * sizeof(float));
       k = ir_solve<M, 0, false>(A, b, xs, p, P, ex, max_steps, tol);
     }
 #pragma unroll
@@ -309,8 +326,9 @@ hipError_t launch_varlen_ir(const Diags<float> &a, double *x, const int64_t *off
   if (next == nullptr) return hipErrorInvalidValue;
   const hipError_t e = launch_fill_int(next, 0, st);
   if (e != hipSuccess) return e;
-  // largest exchange: 10 floats or 4 doubles per thread
-  const size_t smem = (size_t)10 * VL_BLOCK * sizeof(float);
+  // largest exchange: 10 floats or 4 doubles per thread; then the slots of
+  // the system maxima (one float per wavefront)
+  const size_t smem = (size_t)10 * VL_BLOCK * sizeof(float) + (VL_BLOCK / 64) * sizeof(float);
   const int64_t grid = vl_grid(n_items, k_varlen_ir<VL_ROWS>, smem);
   hipLaunchKernelGGL((k_varlen_ir<VL_ROWS>), dim3((unsigned)grid), dim3(VL_BLOCK), smem, st,
                      a.ds, a.dl, a.d, a.du, a.dw, x, offs, perm, items, n_items, next,
