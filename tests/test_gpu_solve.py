@@ -387,3 +387,62 @@ def test_long_x_lines_full_batch():
         with P.Handle() as h:
             check(gen(dims, 3), dims, 0, h)
             assert h.kernel_config(0) == "transpose rows=16 lanes=512", h.kernel_config(0)
+
+
+@pytest.mark.parametrize("dims", [[256, 20, 6], [512, 8, 4], [64, 33, 17], [1024, 5, 3]])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_arrays_off_16_byte_boundaries(dims, dtype):
+    """A caller's arrays that start one element past a 16-byte boundary
+    (views into a larger allocation): x lines leave the 16-byte-vector
+    kernels for ones that load element by element, and every direction
+    matches the oracle with the contents around the views untouched."""
+    arrs = gen(dims, 51, dtype)
+    n = int(np.prod(dims))
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    for sd in range(3):
+        bufs = []
+        for a in arrs:
+            buf = torch.full((n + 2,), 7.0, dtype=tdt, device="cuda")
+            buf[1:n + 1].copy_(torch.from_numpy(a))
+            bufs.append(buf)
+        views = [b[1:n + 1] for b in bufs]
+        assert all(v.data_ptr() % 16 != 0 for v in views)
+        print(f"unaligned dims={dims} sd={sd} {np.dtype(dtype).name}", flush=True)
+        with P.Handle() as h:
+            P.solve(*views[:5], views[5], dims, sd, handle=h)
+            torch.cuda.synchronize()
+            cfg = h.kernel_config(sd)
+        x = views[5].cpu().numpy()
+        err = relerr(x, oracle_solve(arrs, dims, sd))
+        print(f"  err={err:.2e} cfg={cfg}", flush=True)
+        assert err < (FP64_TOL if dtype == np.float64 else FP32_TOL), (sd, cfg, err)
+        for b in bufs:
+            assert b[0].item() == 7.0 and b[n + 1].item() == 7.0
+
+
+@pytest.mark.parametrize("dims", [[256, 20, 6], [512, 8, 4], [1024, 5, 3]])
+def test_mixed_precision_arrays_off_16_byte_boundaries(dims):
+    """The mixed-precision entry point on FP32 coefficient views and an FP64
+    right-hand side view one element past 16-byte boundaries: every direction
+    within 1e-12 of the FP64 oracle of the widened coefficients."""
+    arrs = gen(dims, 52)
+    c32 = [a.astype(np.float32) for a in arrs[:5]]
+    n = int(np.prod(dims))
+    for sd in range(3):
+        cv = []
+        for a in c32:
+            buf = torch.zeros(n + 1, dtype=torch.float32, device="cuda")
+            buf[1:].copy_(torch.from_numpy(a))
+            cv.append(buf[1:])
+        xb = torch.zeros(n + 1, dtype=torch.float64, device="cuda")
+        xb[1:].copy_(torch.from_numpy(arrs[5]))
+        xv = xb[1:]
+        print(f"ir unaligned dims={dims} sd={sd}", flush=True)
+        with P.Handle() as h:
+            P.solve_ir(*cv, xv, dims, sd, handle=h)
+            torch.cuda.synchronize()
+            cfg = h.kernel_config(sd)
+        ref = O.solve(*[a.astype(np.float64) for a in c32], arrs[5], dims, sd)
+        err = relerr(xv.cpu().numpy(), ref)
+        print(f"  err={err:.2e} cfg={cfg}", flush=True)
+        assert err < FP64_TOL, (sd, cfg, err)
