@@ -102,15 +102,16 @@ __global__ void __launch_bounds__(256, (M <= 8 ? 3 : 2))
 // more than half a CU's.  SP = false recomputes the separator spikes in every
 // apply instead of keeping them (ir_core.hpp), which the larger chunk needs
 // to stay in registers; SR is the number of stored PCR rounds (IrFactors).
-// IL: the five coefficient arrays interleaved per 4 rows in one array (ds
-// holds [s0..3, l0..3, d0..3, u0..3, w0..3] per group of 4 rows, the other
-// four pointers are unused): one stream of 80-byte groups instead of five
-// streams, the layout the ADI driver writes when it permutes the
-// coefficients.  Needs M % 4 == 0 and line starts at multiples of 4 rows.
+// IL: the five coefficient arrays interleaved per group of IL rows in one
+// array (IL = 4: ds holds [s0..3, l0..3, d0..3, u0..3, w0..3] per 4 rows;
+// IL = 2: [s0, s1, l0, l1, d0, d1, u0, u1, w0, w1] per 2 rows; the other four
+// pointers are unused): one coefficient stream instead of five, the layout
+// the ADI driver writes when it permutes the coefficients.  Needs M % IL ==
+// 0 and line starts at multiples of IL rows; IL = 0: five arrays.
 // X16: the 16-system tile without pads (tile_put_x16), for tiles that
 // would not fit two workgroups per CU with them.
 template <int M, int R, int Q, int BLOCK = 256, bool SP = true, int SR = ir_stored_rounds(R),
-          bool IL = false, bool X16 = false>
+          int IL = 0, bool X16 = false>
 __global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     k_ir_xq(const float *__restrict__ ds, const float *__restrict__ dl,
             const float *__restrict__ d, const float *__restrict__ du,
@@ -134,7 +135,20 @@ __global__ void __launch_bounds__(BLOCK, 512 / BLOCK)
     const int64_t off = slo * map.in_lo + shi * map.in_hi + (int64_t)p * M;
     const int64_t offc = slo * map.c_lo + shi * map.c_hi + (int64_t)p * M;
     float s[M], l[M], dd[M], u[M], w[M];
-    if constexpr (IL) {
+    if constexpr (IL == 2) {
+      static_assert(M % 2 == 0, "interleaved coefficients come in groups of 2 rows");
+      const float2 *g = reinterpret_cast<const float2 *>(ds) + (offc >> 1) * 5;
+#pragma unroll
+      for (int k = 0; k < M / 2; ++k) {
+        const float2 a0 = g[5 * k], a1 = g[5 * k + 1], a2 = g[5 * k + 2], a3 = g[5 * k + 3],
+                     a4 = g[5 * k + 4];
+        s[2 * k] = a0.x; s[2 * k + 1] = a0.y;
+        l[2 * k] = a1.x; l[2 * k + 1] = a1.y;
+        dd[2 * k] = a2.x; dd[2 * k + 1] = a2.y;
+        u[2 * k] = a3.x; u[2 * k + 1] = a3.y;
+        w[2 * k] = a4.x; w[2 * k + 1] = a4.y;
+      }
+    } else if constexpr (IL == 4) {
       static_assert(M % 4 == 0, "interleaved coefficients come in groups of 4 rows");
       const float4 *g = reinterpret_cast<const float4 *>(ds) + (offc >> 2) * 5;
 #pragma unroll
@@ -440,10 +454,10 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
     const int64_t grid = (n_sys + per_wg - 1) / per_wg;
     const size_t smem = sizeof(double) * 16 * (size_t)len;
     const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&k_ir_xq<M, R, Q, 256, false, 2, false, true>),
+        reinterpret_cast<const void *>(&k_ir_xq<M, R, Q, 256, false, 2, 0, true>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ir_xq<M, R, Q, 256, false, 2, false, true>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((k_ir_xq<M, R, Q, 256, false, 2, 0, true>), dim3((unsigned)grid),
                        dim3(block), smem, st, a.ds, a.dl, a.d, a.du, a.dw, b, out, n_sys, len,
                        max_steps, tol, steps, map);
     return hipGetLastError();
@@ -494,31 +508,40 @@ hipError_t go_ir_x(const Diags<float> &a, const double *b, double *out, bool rot
   return hipGetLastError();
 }
 
-// the rotated-store kernel over interleaved coefficients (k_ir_xq, IL)
+// the rotated-store kernel over interleaved coefficients (k_ir_xq, IL):
+// groups of 4 rows for 4- and 8-row chunks, of 2 rows for 6-row chunks and
+// for the 10-row chunks of 640-row lines (the unpadded 16-system tile)
 template <int M, int R>
 hipError_t go_ir_x_il(const float *coef, const double *b, double *out, int64_t n_sys,
                       int64_t len, const XMap &map, int max_steps, double tol, int *steps,
                       hipStream_t st) {
-  if constexpr (M % 4 != 0) {
-    return hipErrorInvalidValue;
-  } else {
-    const int block = 256;
+  const int block = 256;
+  auto go = [&](auto kernel, int64_t per_wg, size_t smem) {
+    if (smem > 65536) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    const int64_t grid = (n_sys + per_wg - 1) / per_wg;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), smem, st, coef, coef, coef,
+                       coef, coef, b, out, n_sys, len, max_steps, tol, steps, map);
+    return hipGetLastError();
+  };
+  if constexpr (M == 10) {
+    if constexpr (R == 6)
+      return go(k_ir_xq<M, R, 4, 256, false, 2, 2, true>, 16, sizeof(double) * 16 * (size_t)len);
+    else
+      return hipErrorInvalidValue;
+  } else if constexpr (M <= 8) {
+    constexpr int G = M % 4 == 0 ? 4 : 2;
     constexpr int run = (256 >> R) * 8;
     constexpr int Q = run >= 128 ? 1 : 4;
     constexpr int SR = ir_stored_rounds(R);
-    const int64_t per_wg = (int64_t)(256 >> R) * Q;
-    const int64_t grid = (n_sys + per_wg - 1) / per_wg;
-    const size_t smem = rotated_tile_bytes<double>(block * Q, M, len);
-    if (smem > 65536) {
-      const hipError_t e = hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&k_ir_xq<M, R, Q, 256, true, SR, true>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_ir_xq<M, R, Q, 256, true, SR, true>), dim3((unsigned)grid),
-                       dim3(block), smem, st, coef, coef, coef, coef, coef, b, out, n_sys, len,
-                       max_steps, tol, steps, map);
-    return hipGetLastError();
+    return go(k_ir_xq<M, R, Q, 256, true, SR, G>, (int64_t)(256 >> R) * Q,
+              rotated_tile_bytes<double>(block * Q, M, len));
+  } else {
+    return hipErrorInvalidValue;
   }
 }
 
@@ -577,23 +600,29 @@ hipError_t launch_ir_x_map(const Diags<float> &a, const double *b, double *out,
   return hipErrorInvalidValue;
 }
 
-bool ir_x_interleave_ok(int rows, int64_t len) {
-  return (rows == 4 || rows == 8) && ir_x_supported(rows, len) && len % 4 == 0;
+int ir_x_interleave_group(int rows, int64_t len) {
+  if (!ir_x_supported(rows, len)) return 0;
+  if (rows == 4 || rows == 8) return len % 4 == 0 ? 4 : 0;
+  if (rows == 6) return 2;
+  if (rows == 10) return len == 640 ? 2 : 0; // 64 lanes: the unpadded-tile kernel
+  return 0;
 }
 
 hipError_t launch_ir_x_map_il(const float *coef, const double *b, double *out, int64_t n_sys,
                               int64_t len, const XMap &map, int rows, int max_steps, double tol,
                               int *steps, hipStream_t st) {
-  // every line must start on a group of 4 rows
-  if (!ir_x_interleave_ok(rows, len) || map.c_lo % 4 || map.c_hi % 4 ||
-      (reinterpret_cast<uintptr_t>(coef) & 15u))
+  // every line must start on a group of rows
+  const int G = ir_x_interleave_group(rows, len);
+  if (G == 0 || map.c_lo % G || map.c_hi % G || (reinterpret_cast<uintptr_t>(coef) & 15u))
     return hipErrorInvalidValue;
   const int R = ceil_log2(len / rows);
 #define IRXI(MM, RR) \
   if (rows == MM && R == RR) \
     return go_ir_x_il<MM, RR>(coef, b, out, n_sys, len, map, max_steps, tol, steps, st);
   IRXI(4, 1) IRXI(4, 2) IRXI(4, 3) IRXI(4, 4) IRXI(4, 5) IRXI(4, 6)
+  IRXI(6, 1) IRXI(6, 2) IRXI(6, 3) IRXI(6, 4) IRXI(6, 5) IRXI(6, 6)
   IRXI(8, 1) IRXI(8, 2) IRXI(8, 3) IRXI(8, 4) IRXI(8, 5) IRXI(8, 6)
+  IRXI(10, 6)
 #undef IRXI
   return hipErrorInvalidValue;
 }

@@ -137,23 +137,50 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-hipError_t launch_interleave4(const float *const src[5], float *out, int64_t n,
-                              hipStream_t st) {
-  if (n % 4) return hipErrorInvalidValue;
+// out[10 * g + 2 * k + r] = src[k][2 * g + r]: the same per group of 2 rows
+// (8-byte pieces), for chunks of 6 or 10 rows.
+__global__ void __launch_bounds__(256)
+    k_interleave2(const float2 *__restrict__ s0, const float2 *__restrict__ s1,
+                  const float2 *__restrict__ s2, const float2 *__restrict__ s3,
+                  const float2 *__restrict__ s4, float2 *__restrict__ out, int64_t groups) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    float2 *o = out + 5 * g;
+    o[0] = s0[g];
+    o[1] = s1[g];
+    o[2] = s2[g];
+    o[3] = s3[g];
+    o[4] = s4[g];
+  }
+}
+
+hipError_t launch_interleave(const float *const src[5], float *out, int64_t n, int group,
+                             hipStream_t st) {
+  if ((group != 2 && group != 4) || n % group) return hipErrorInvalidValue;
+  const uintptr_t align = (uintptr_t)group * sizeof(float) - 1;
   for (int k = 0; k < 5; ++k)
-    if (reinterpret_cast<uintptr_t>(src[k]) & 15u) return hipErrorInvalidValue;
-  if (reinterpret_cast<uintptr_t>(out) & 15u) return hipErrorInvalidValue;
-  const int64_t groups = n / 4;
+    if (reinterpret_cast<uintptr_t>(src[k]) & align) return hipErrorInvalidValue;
+  if (reinterpret_cast<uintptr_t>(out) & align) return hipErrorInvalidValue;
+  const int64_t groups = n / group;
   const int block = 256;
   int64_t grid = (groups + block - 1) / block;
   if (grid > 256 * 64) grid = 256 * 64;
-  hipLaunchKernelGGL(k_interleave4, dim3((unsigned)grid), dim3(block), 0, st,
-                     reinterpret_cast<const float4 *>(src[0]),
-                     reinterpret_cast<const float4 *>(src[1]),
-                     reinterpret_cast<const float4 *>(src[2]),
-                     reinterpret_cast<const float4 *>(src[3]),
-                     reinterpret_cast<const float4 *>(src[4]), reinterpret_cast<float4 *>(out),
-                     groups);
+  if (group == 4)
+    hipLaunchKernelGGL(k_interleave4, dim3((unsigned)grid), dim3(block), 0, st,
+                       reinterpret_cast<const float4 *>(src[0]),
+                       reinterpret_cast<const float4 *>(src[1]),
+                       reinterpret_cast<const float4 *>(src[2]),
+                       reinterpret_cast<const float4 *>(src[3]),
+                       reinterpret_cast<const float4 *>(src[4]), reinterpret_cast<float4 *>(out),
+                       groups);
+  else
+    hipLaunchKernelGGL(k_interleave2, dim3((unsigned)grid), dim3(block), 0, st,
+                       reinterpret_cast<const float2 *>(src[0]),
+                       reinterpret_cast<const float2 *>(src[1]),
+                       reinterpret_cast<const float2 *>(src[2]),
+                       reinterpret_cast<const float2 *>(src[3]),
+                       reinterpret_cast<const float2 *>(src[4]), reinterpret_cast<float2 *>(out),
+                       groups);
   return hipGetLastError();
 }
 

@@ -77,9 +77,9 @@ struct Adi {
   int fbytes = 8; // field element size
   bool rotating = true;
   void *coef[3][5] = {};
-  // mode 2: direction d's five FP32 arrays interleaved per 4 rows at
-  // coef[d][0] (one 80-byte group per 4 rows instead of five streams)
-  bool il[3] = {false, false, false};
+  // mode 2: direction d's five FP32 arrays interleaved per il[d] rows (4 or
+  // 2; 0: five arrays) at coef[d][0] (one coefficient stream instead of five)
+  int il[3] = {0, 0, 0};
   void *coef_mem = nullptr;
   void *tab[3] = {}; // shared-fact factor tables (modes 3, 4), in coef_mem
   void *scale[3] = {}; // structured coefficients (modes 5, 6, 7), in coef_mem
@@ -455,25 +455,26 @@ pentadStatus_t pentadsolver_adi_create(pentadsolver_handle_t h,
     }
   }
   // refinement mode on rotated layouts: the five permuted arrays of a
-  // direction whose kernel takes 4- or 8-row chunks are rewritten
-  // interleaved per 4 rows into the same 5 * n floats (the 3-stream byte mix
+  // direction whose kernel has an interleaved form are rewritten interleaved
+  // per group of rows into the same 5 * n floats (the 3-stream byte mix
   // streams about 2% faster than five separate coefficient streams)
   if (mode == 2 && A->rotating) {
     void *tmp = nullptr;
     for (int d = 0; d < ndim; ++d) {
-      if (A->plan[d].kind != Plan::IrX || !ir_x_interleave_ok(A->plan[d].rows, A->dims[d]) ||
-          A->n % 4)
-        continue;
+      const int G = A->plan[d].kind == Plan::IrX
+                        ? ir_x_interleave_group(A->plan[d].rows, A->dims[d])
+                        : 0;
+      if (G == 0 || A->n % G) continue;
       if (!tmp && hipMalloc(&tmp, csz * 5) != hipSuccess) return fail(PENTAD_STATUS_ALLOC_FAILED);
       const float *src[5];
       for (int k = 0; k < 5; ++k) src[k] = (const float *)A->coef[d][k];
-      if (launch_interleave4(src, (float *)tmp, A->n, st) != hipSuccess ||
+      if (launch_interleave(src, (float *)tmp, A->n, G, st) != hipSuccess ||
           hipMemcpyAsync(A->coef[d][0], tmp, csz * 5, hipMemcpyDeviceToDevice, st) != hipSuccess) {
         (void)hipStreamSynchronize(st);
         (void)hipFree(tmp);
         return fail(PENTAD_STATUS_EXECUTION_FAILED);
       }
-      A->il[d] = true;
+      A->il[d] = G;
     }
     if (tmp) {
       const hipError_t e = hipStreamSynchronize(st);

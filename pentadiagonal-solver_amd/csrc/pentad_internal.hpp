@@ -218,9 +218,11 @@ hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int r
                         int sys_per_wg, int max_steps, double tol, int *steps,
                         hipStream_t st);
 // The rotated-store refinement kernel reading the five coefficient arrays
-// interleaved per 4 rows in one array (the layout launch_interleave4 writes):
-// 4- and 8-row chunks, lines of a multiple of 4 rows starting at multiples of
-// 4 elements, coef 16-byte aligned.
+// interleaved in one array (the layout launch_interleave writes) per group
+// of ir_x_interleave_group(rows, len) rows: 4 for 4- and 8-row chunks
+// (lines of a multiple of 4 rows), 2 for 6-row chunks and for 640-row lines
+// of 10-row chunks, 0 where the kernel has no interleaved form; line starts
+// at multiples of the group, coef 16-byte aligned.
 // The x kernels over FP32 coefficients widened to FP64 on load, FP64 b and
 // out: one FP64 solve of the system the refinement converges to, for the
 // lengths no refinement kernel covers (the rows / lanes of
@@ -228,13 +230,14 @@ hipError_t launch_ir_yz(const Diags<float> &a, double *x, const Lines &ln, int r
 hipError_t launch_part_x_map_widened(const Diags<float> &a, const double *b, double *out,
                                      bool rot, int64_t n_sys, int64_t len, const XMap &map,
                                      int rows, int lanes, hipStream_t st);
-bool ir_x_interleave_ok(int rows, int64_t len);
+int ir_x_interleave_group(int rows, int64_t len);
 hipError_t launch_ir_x_map_il(const float *coef, const double *b, double *out, int64_t n_sys,
                               int64_t len, const XMap &map, int rows, int max_steps, double tol,
                               int *steps, hipStream_t st);
-// out[20 * (i / 4) + 4 * k + i % 4] = src[k][i], k < 5, i < n (n % 4 == 0)
-hipError_t launch_interleave4(const float *const src[5], float *out, int64_t n,
-                              hipStream_t st);
+// out[5 G (i / G) + G k + i % G] = src[k][i], k < 5, i < n, for a group of
+// G = 2 or 4 rows (n % G == 0)
+hipError_t launch_interleave(const float *const src[5], float *out, int64_t n, int group,
+                             hipStream_t st);
 // x direction, any length <= 8 * 64: coefficient spans staged through LDS
 // (rows 8, lanes the next power of two, identity rows past the end)
 bool ir_tr_x_supported(int rows, int64_t len);

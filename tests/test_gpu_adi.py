@@ -135,13 +135,16 @@ def test_adi_multistep_graph_bit_identical_to_eager():
 
 
 @pytest.mark.parametrize("dims,ratio", [([512, 64, 32], (2.0, 3.0)), ([256, 32, 40], (2.0, 3.0)),
-                                        ([512, 16, 24], (1.05, 1.05)), ([384, 256, 8], (2.0, 3.0))])
+                                        ([512, 16, 24], (1.05, 1.05)), ([384, 256, 8], (2.0, 3.0)),
+                                        ([640, 12, 20], (2.0, 3.0)), ([20, 640, 24], (1.05, 1.05)),
+                                        ([320, 24, 12], (2.0, 3.0))])
 def test_adi_ir_interleaved_coefficients(dims, ratio):
-    """Refinement directions of 4- and 8-row chunks read their coefficients
-    interleaved per 4 rows (one 80-byte group per 4 rows, written when the
-    driver permutes them); the others keep five arrays.  Two steps against
-    the oracle (<= 1e-12, well-conditioned and weak sets), and a captured
-    graph replays to the eager result bit for bit."""
+    """Refinement directions read their coefficients interleaved (written
+    when the driver permutes them): per 4 rows for 4- and 8-row chunks, per
+    2 rows for 6-row chunks and for the 10-row chunks of 640-row lines; the
+    others (10-row chunks of other lengths) keep five arrays.  Two steps
+    against the oracle (<= 1e-12, well-conditioned and weak sets), and a
+    captured graph replays to the eager result bit for bit."""
     coef, b = gen(dims, 31, ratio)
     print(f"adi ir interleaved dims={dims} ratio={ratio}", flush=True)
     c32 = [a.astype(np.float32) for a in coef]
@@ -157,9 +160,9 @@ def test_adi_ir_interleaved_coefficients(dims, ratio):
         cfg = [adi.config(k) for k in range(len(dims))]
     err = relerr(xe.cpu().numpy(), ref)
     print(f"  err={err:.2e} cfg={cfg}", flush=True)
-    for c in cfg:
+    for c, n in zip(cfg, dims):
         rows = int(c.split("rows=")[1].split()[0])
-        assert ("interleaved" in c) == (rows in (4, 8)), c
+        assert ("interleaved" in c) == (rows in (4, 6, 8) or (rows == 10 and n == 640)), c
     assert any("interleaved" in c for c in cfg), cfg
     assert err < 1e-12
     assert np.array_equal(xg.cpu().numpy(), xe.cpu().numpy())
@@ -364,5 +367,5 @@ def test_adi_ir_640_rows_partial_tile(dims):
         cfg = [adi.config(k) for k in range(len(dims))]
     err = relerr(got, oracle_steps([[a.astype(np.float64) for a in c32]], b, dims, 2))
     print(f"  err={err:.2e} cfg={cfg}", flush=True)
-    assert cfg[dims.index(640)] == "ir thomas-pcr rows=10 lanes=64", cfg
+    assert cfg[dims.index(640)] == "ir interleaved thomas-pcr rows=10 lanes=64", cfg
     assert err < 1e-12
