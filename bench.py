@@ -881,8 +881,25 @@ def main():
             key = "lanes_coalesced_writes" if layout == 0 else f"kernel_structure_rotated_valu{8 * rounds}"
             probe[key] = {"ms": best, "GBs": 36 * n / (best * 1e-3) / 1e9}
         del cf
+        # the FP64 strided solve's I/O without the solve, y and z lines of
+        # 512 rows: whole lines per workgroup (the drop-in kernel's
+        # structure) and lines shared by 2 or 4 workgroups
+        strided = {}
+        if dims == [512] * 3:
+            xo.zero_()
+            for sd, split in ((1, 1), (2, 1), (2, 2), (2, 4)):
+                best = 1e30
+                for _ in range(10):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(torch.cuda.current_stream())
+                    P.probe_strided_io(split, list(c64), xo, dims, sd, handle=h)
+                    e1.record(torch.cuda.current_stream())
+                    torch.cuda.synchronize()
+                    best = min(best, e0.elapsed_time(e1))
+                strided[f"{'yz'[sd - 1]}_split{split}"] = {
+                    "ms": best, "GBs": 56 * n / (best * 1e-3) / 1e9}
         del xo
-        log(f"bench: refinement byte-mix probe {probe}")
+        log(f"bench: refinement byte-mix probe {probe}; strided I/O probe {strided}")
     log("bench: drop-in in-place FP64 direction solves")
     dropin_ms, dropin_k = dropin_direction_ms(P, torch, c64, b, dims, h)
     log("bench: drop-in in-place mixed-precision direction solves")
@@ -984,6 +1001,7 @@ def main():
     }
     if args.probe:
         result["probe_ir_byte_mix"] = probe
+        result["probe_strided_io"] = strided
     if not args.no_cpu:
         log("bench: cpu baseline (reference CPU solver)")
         result["cpu_baseline"] = cpu_baseline(c64_np, b_np, dims)

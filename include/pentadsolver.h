@@ -304,6 +304,18 @@ pentadStatus_t pentadsolver_probe_ir_mix(pentadsolver_handle_t handle, int strea
 pentadStatus_t pentadsolver_probe_ir_layout(pentadsolver_handle_t handle, int layout,
                                             int rounds, const float *coef, const double *b,
                                             double *x, size_t n);
+/* Bandwidth probe with the FP64 strided solve's I/O: along t_solvedim (1 = y
+ * or 2 = z of a 3-D grid t_dims, t_dims[t_solvedim] = 512, the product of
+ * the other two a multiple of 64) every line's rows of the five diagonals
+ * and of x are read and x += ds + dl + d + du + dw is written in place, by
+ * 512-thread workgroups of 16 * split adjacent lines x 512 / split rows
+ * (split 1: whole lines, the drop-in kernel's structure; 2 or 4: each line
+ * shared by 2 or 4 workgroups), every row loaded before any is stored, as
+ * in the solve.  No reference counterpart (a variant of
+ * apps/roofline_probe_cuda.cu:27-49). */
+pentadStatus_t pentadsolver_probe_strided_io(pentadsolver_handle_t handle, int split,
+                                             const double *const coef[5], double *x,
+                                             const int *t_dims, int t_solvedim);
 
 /* ADI sweep driver.  One step solves along every direction in turn (x, then
  * y, then z for ndim = 3) in place on a field x in the standard layout

@@ -68,6 +68,7 @@ _SIGNATURES = {
     "pentadsolver_probe_stream": (_I, [_P, ctypes.POINTER(_P), _P, _SZ, _I, _I]),
     "pentadsolver_probe_ir_mix": (_I, [_P, _I, ctypes.POINTER(_P), _P, _P, _SZ]),
     "pentadsolver_probe_ir_layout": (_I, [_P, _I, _I, _P, _P, _P, _SZ]),
+    "pentadsolver_probe_strided_io": (_I, [_P, _I, ctypes.POINTER(_P), _P, _IP, _I]),
     "pentadsolver_adi_create": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), _I, _I, _I, ctypes.c_double]),
     "pentadsolver_adi_create_stencil": (_I, [_P, ctypes.POINTER(_P), _I, _I, _IP, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "pentadsolver_D_gpsv_stencil_batch": (_I, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_double, _P, _IP, _I, _I]),
@@ -432,6 +433,17 @@ def probe_ir_layout(layout, rounds, coef, b, x, handle=None):
     float32 tensor of 5n floats (interleaved per 4 rows)."""
     _check(load_library().pentadsolver_probe_ir_layout(
         _hval(handle), int(layout), int(rounds), _ptr(coef), _ptr(b), _ptr(x), x.numel()))
+
+
+def probe_strided_io(split, coef, x, dims, solvedim, handle=None):
+    """The FP64 strided solve's I/O without the solve, lines of 512 rows along
+    y (solvedim 1) or z (2) of a 3-D grid: x += sum of the five float64
+    diagonals in place, by workgroups of 16 * split lines x 512 / split
+    rows (split 1, 2 or 4)."""
+    arr = (_P * 5)(*[_ptr(t) for t in coef])
+    d = (ctypes.c_int * 3)(*dims)
+    _check(load_library().pentadsolver_probe_strided_io(
+        _hval(handle), int(split), arr, _ptr(x), d, int(solvedim)))
 
 
 ADI_FP64, ADI_FP32, ADI_IR, ADI_SHARED_FP64, ADI_SHARED_FP32 = 0, 1, 2, 3, 4

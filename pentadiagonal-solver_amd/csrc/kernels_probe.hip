@@ -10,6 +10,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "partition_exchange.hpp"
+
 namespace pentad {
 
 template <typename T>
@@ -199,6 +201,57 @@ hipError_t launch_probe_ir_layout(int layout, int rounds, const float *coef, con
     hipLaunchKernelGGL(k_probe_ir_layout<true>, dim3((unsigned)(n / (16 * 512))), dim3(256),
                        smem, st, c, b2, x, rounds);
   }
+  return hipGetLastError();
+}
+
+// The strided directions' I/O without the solve, lines of 512 FP64 rows:
+// a 512-thread workgroup holds S adjacent systems x P chunks of 16 rows,
+// loads every row of the five diagonals and x into registers, waits at a
+// barrier for the whole workgroup's loads (the solve exchanges through LDS
+// before it stores anything), then writes x += ds + dl + d + du + dw in
+// place.  S = 16 (K = 1) is the FP64 drop-in kernel's structure, whole lines
+// per workgroup; S = 32 / 64 gives each workgroup half / a quarter of every
+// line (K = 2 / 4 workgroups per line set), so that one visit of a 2 MB page
+// of a z line (stride 512^2 elements) serves 256 / 512 B per array instead
+// of 128.  System sys starts at (sys / lo) * lo * 512 + sys % lo with row
+// stride lo (lo = the extent below the solve axis: nx for y, nx * ny for z).
+// Tiles are dealt to the XCDs as the solve kernel deals them (xcd_tile), the
+// K workgroups of one line set and the neighbouring line sets on one XCD.
+template <int S>
+__global__ void __launch_bounds__(512, 1)
+    k_probe_strided_io(const double *__restrict__ a0, const double *__restrict__ a1,
+                       const double *__restrict__ a2, const double *__restrict__ a3,
+                       const double *__restrict__ a4, double *__restrict__ x, int64_t lo) {
+  constexpr int M = 16, P = 512 / S, K = S / 16;
+  const int t = (int)threadIdx.x, s = t % S, c = t / S;
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t g = blk / K;
+  const int k = (int)(blk % K);
+  const int64_t sys = g * S + s;
+  const int64_t base = (sys / lo) * lo * 512 + sys % lo + ((int64_t)k * P * M + (int64_t)c * M) * lo;
+  const double *src[6] = {a0, a1, a2, a3, a4, x};
+  double v[6][M];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+#pragma unroll
+    for (int i = 0; i < M; ++i) v[q][i] = src[q][base + (int64_t)i * lo];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+    x[base + (int64_t)i * lo] = v[5][i] + v[0][i] + v[1][i] + v[2][i] + v[3][i] + v[4][i];
+}
+
+hipError_t launch_probe_strided_io(int split, const double *const a[5], double *x, int64_t lo,
+                                   int64_t n_sys, hipStream_t st) {
+  const int S = 16 * split;
+  if ((split != 1 && split != 2 && split != 4) || n_sys % S != 0) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(n_sys / S * split)), block(512);
+  if (split == 1)
+    hipLaunchKernelGGL(k_probe_strided_io<16>, grid, block, 0, st, a[0], a[1], a[2], a[3], a[4], x, lo);
+  else if (split == 2)
+    hipLaunchKernelGGL(k_probe_strided_io<32>, grid, block, 0, st, a[0], a[1], a[2], a[3], a[4], x, lo);
+  else
+    hipLaunchKernelGGL(k_probe_strided_io<64>, grid, block, 0, st, a[0], a[1], a[2], a[3], a[4], x, lo);
   return hipGetLastError();
 }
 

@@ -26,6 +26,8 @@ hipError_t launch_probe(const void *const in[6], void *out, size_t n,
                         int elem_bytes, int fmas, hipStream_t st);
 hipError_t launch_probe_ir_mix(int streams, const float *const coef[5], const double *b,
                                double *x, size_t n, hipStream_t st);
+hipError_t launch_probe_strided_io(int split, const double *const a[5], double *x, int64_t lo,
+                                   int64_t n_sys, hipStream_t st);
 hipError_t launch_probe_ir_layout(int layout, int rounds, const float *coef, const double *b,
                                   double *x, size_t n, hipStream_t st);
 
@@ -362,6 +364,24 @@ pentadStatus_t pentadsolver_probe_ir_layout(pentadsolver_handle_t h, int layout,
   if (layout == 0 ? (n % 2048 != 0) : (n != (size_t)512 * 512 * 512))
     return PENTAD_STATUS_INVALID_VALUE;
   const hipError_t e = launch_probe_ir_layout(layout, rounds, coef, b, x, n, ctx_of(h)->stream);
+  return e == hipSuccess ? PENTAD_STATUS_SUCCESS : PENTAD_STATUS_EXECUTION_FAILED;
+}
+
+pentadStatus_t pentadsolver_probe_strided_io(pentadsolver_handle_t h, int split,
+                                             const double *const coef[5], double *x,
+                                             const int *t_dims, int t_solvedim) {
+  if (!coef || !x || !t_dims || (t_solvedim != 1 && t_solvedim != 2) ||
+      (split != 1 && split != 2 && split != 4))
+    return PENTAD_STATUS_INVALID_VALUE;
+  for (int i = 0; i < 5; ++i)
+    if (!coef[i]) return PENTAD_STATUS_INVALID_VALUE;
+  for (int i = 0; i < 3; ++i)
+    if (t_dims[i] < 1) return PENTAD_STATUS_INVALID_VALUE;
+  if (t_dims[t_solvedim] != 512) return PENTAD_STATUS_INVALID_VALUE;
+  const int64_t lo = t_solvedim == 1 ? (int64_t)t_dims[0] : (int64_t)t_dims[0] * t_dims[1];
+  const int64_t n_sys = (int64_t)t_dims[0] * t_dims[1] * t_dims[2] / 512;
+  if (n_sys % 64 != 0) return PENTAD_STATUS_INVALID_VALUE;
+  const hipError_t e = launch_probe_strided_io(split, coef, x, lo, n_sys, ctx_of(h)->stream);
   return e == hipSuccess ? PENTAD_STATUS_SUCCESS : PENTAD_STATUS_EXECUTION_FAILED;
 }
 
