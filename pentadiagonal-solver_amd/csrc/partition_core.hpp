@@ -81,9 +81,24 @@ __device__ __forceinline__ bool pcr_negligible(const M2<T> &A, const M2<T> &C) {
 // groups; on return (r0, r1) = S_p.  Round h = 1 is peeled so that its
 // exchange is a compile-time lane shift; later rounds stop once the
 // couplings are negligible on every lane (pcr_negligible).
-template <typename T, typename Ex>
-__device__ __forceinline__ void block_pcr(M2<T> &A, M2<T> &C, T &r0, T &r1, int P, Ex &ex) {
+// EXT: every block row also carries a 2x2 right-hand side E for an unknown
+// pair X from outside the group (S_p + A S_{p-1} + C S_{p+1} = r + E X);
+// the rounds reduce E as they reduce r, so on return S_p = r + E X.
+template <typename T, bool EXT, typename Ex>
+__device__ __forceinline__ void block_pcr_impl(M2<T> &A, M2<T> &C, T &r0, T &r1, M2<T> &E,
+                                               int P, Ex &ex) {
   auto round = [&](int h) {
+    // EXT: E - A Ep - C En first, through an exchange of its own, so that
+    // the neighbours' E and A, C, r are not in registers together
+    M2<T> Eq{};
+    if constexpr (EXT) {
+      T se[4] = {E.a00, E.a01, E.a10, E.a11}, ep[4], en[4];
+      ex.template swapc<4>(se, ep, en, h);
+      const M2<T> AEp = mul(A, M2<T>{ep[0], ep[1], ep[2], ep[3]});
+      const M2<T> CEn = mul(C, M2<T>{en[0], en[1], en[2], en[3]});
+      Eq = M2<T>{E.a00 - AEp.a00 - CEn.a00, E.a01 - AEp.a01 - CEn.a01,
+                 E.a10 - AEp.a10 - CEn.a10, E.a11 - AEp.a11 - CEn.a11};
+    }
     T send[10], rp[10], rn[10];
     send[0] = A.a00; send[1] = A.a01; send[2] = A.a10; send[3] = A.a11;
     send[4] = C.a00; send[5] = C.a01; send[6] = C.a10; send[7] = C.a11;
@@ -101,6 +116,7 @@ __device__ __forceinline__ void block_pcr(M2<T> &A, M2<T> &C, T &r0, T &r1, int 
                  (C.a10 * rn[8] + C.a11 * rn[9]);
     M2<T> An2 = mul(A, Ap), Cn2 = mul(C, Cn);
     const M2<T> Bi = inv(Bn);
+    if constexpr (EXT) E = mul(Bi, Eq); // normalised like r
     const M2<T> nA{-An2.a00, -An2.a01, -An2.a10, -An2.a11};
     const M2<T> nC{-Cn2.a00, -Cn2.a01, -Cn2.a10, -Cn2.a11};
     A = mul(Bi, nA);
@@ -115,23 +131,31 @@ __device__ __forceinline__ void block_pcr(M2<T> &A, M2<T> &C, T &r0, T &r1, int 
   }
 }
 
-// Solve the chunk owned by this lane.  On entry s,l,d,u,w,f hold rows
-// r0..r0+M-1 of the five diagonals and the RHS; on return f holds the
-// solution of those rows.  p is the chunk index, P the chunk count.
-template <typename T, int M, typename Ex>
-__device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
-                                           const T (&d)[M], const T (&u)[M],
-                                           const T (&w)[M], T (&f)[M], int p,
-                                           int P, Ex &ex) {
-  static_assert(M >= 4, "a chunk needs at least two interior rows");
-  constexpr int m = M - 2; // interior rows 0..m-1, separator rows m, m+1
-  const bool first = (p == 0);
-  const bool last = (p == P - 1);
+template <typename T, typename Ex>
+__device__ __forceinline__ void block_pcr(M2<T> &A, M2<T> &C, T &r0, T &r1, int P, Ex &ex) {
+  M2<T> none{};
+  block_pcr_impl<T, false>(A, C, r0, r1, none, P, ex);
+}
 
+// Steps 1 and 2 of the solve below for one chunk: the interior eliminated
+// with three right-hand sides (du2, dw2, zf, za, zb: row i reads
+// x_i + du2_i x_{i+1} + dw2_i x_{i+2} = zf_i + za_i s0 + zb_i s1), then its
+// last two rows (XL2, XL1) and first two rows (XF0, XF1) as affine functions
+// [f, s0, s1, t0, t1] of the previous separator (s0, s1) and the chunk's own
+// separator (t0, t1).  first: the chunk starts the system (no previous
+// separator).
+template <typename T, int M>
+__device__ __forceinline__ void chunk_eliminate(const T (&s)[M], const T (&l)[M],
+                                                const T (&d)[M], const T (&u)[M],
+                                                const T (&w)[M], const T (&f)[M], bool first,
+                                                T (&du2)[M - 2], T (&dw2)[M - 2],
+                                                T (&zf)[M - 2], T (&za)[M - 2],
+                                                T (&zb)[M - 2], T (&XL1)[5], T (&XL2)[5],
+                                                T (&XF0)[5], T (&XF1)[5]) {
+  constexpr int m = M - 2;
   // ---- 1. forward elimination of the interior, 3 right-hand sides --------
   // After it, x_i + du2_i x_{i+1} + dw2_i x_{i+2} = zf_i + za_i s0 + zb_i s1
   // where (s0, s1) = S_{p-1}.
-  T du2[m], dw2[m], zf[m], za[m], zb[m];
   {
     const T iv = recip(d[0]);
     du2[0] = u[0] * iv;
@@ -163,7 +187,6 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
   // ---- 2. backward sweep with 5 components [f, s0, s1, t0, t1] -----------
   // (t0, t1) = S_p = (x_m, x_{m+1}).  Keep rows m-2, m-1 (XL2, XL1) and
   // rows 0, 1 (XF0, XF1).
-  T XL1[5], XL2[5], XF0[5], XF1[5];
   {
     const int i = m - 1;
     XL1[0] = zf[i];
@@ -205,6 +228,81 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
       XF1[c] = xb[c];
     }
   }
+}
+
+// Step 4 for one chunk: with the previous separator (s0, s1) and the
+// chunk's own (t0, t1) known, the backward sweep over the interior; f
+// receives the solution of all M rows.
+template <typename T, int M>
+__device__ __forceinline__ void chunk_backsub(const T (&du2)[M - 2], const T (&dw2)[M - 2],
+                                              const T (&zf)[M - 2], const T (&za)[M - 2],
+                                              const T (&zb)[M - 2], T s0, T s1, T t0, T t1,
+                                              T (&f)[M]) {
+  constexpr int m = M - 2;
+  T xa = t0, xb = t1; // x_{i+1}, x_{i+2}
+#pragma unroll
+  for (int i = m - 1; i >= 0; --i) {
+    const T hi = zf[i] + za[i] * s0 + zb[i] * s1;
+    const T xi = hi - du2[i] * xa - dw2[i] * xb;
+    f[i] = xi;
+    xb = xa;
+    xa = xi;
+  }
+  f[m] = t0;
+  f[m + 1] = t1;
+}
+
+// Step 3 for one chunk: its separator equations (rows m and m+1) as the
+// block row A S_{p-1} + B S_p + C S_{p+1} = r, with the next chunk's first
+// two rows Y0, Y1 (affine functions [f, s0, s1, t0, t1] of S_p and S_{p+1})
+// substituted where rows m, m+1 reach past the chunk: wa = w_m, ub =
+// u_{m+1}, wb = w_{m+1}, or zeros where those couplings do not go to the
+// next chunk of the group.
+template <typename T, int M>
+__device__ __forceinline__ void separator_block_row(const T (&s)[M], const T (&l)[M],
+                                                    const T (&d)[M], const T (&u)[M],
+                                                    const T (&f)[M], const T (&XL1)[5],
+                                                    const T (&XL2)[5], const T (&Y0)[5],
+                                                    const T (&Y1)[5], T wa, T ub, T wb,
+                                                    M2<T> &A, M2<T> &B, M2<T> &C, T &r0,
+                                                    T &r1) {
+  constexpr int m = M - 2;
+  const T sa = s[m], la = l[m], da = d[m], ua = u[m];
+  const T sb = s[m + 1], lb = l[m + 1], db = d[m + 1];
+  // eq a: sa x_{m-2} + la x_{m-1} + da t0 + ua t1 + wa y0 = f_m
+  A.a00 = sa * XL2[1] + la * XL1[1];
+  A.a01 = sa * XL2[2] + la * XL1[2];
+  B.a00 = sa * XL2[3] + la * XL1[3] + da + wa * Y0[1];
+  B.a01 = sa * XL2[4] + la * XL1[4] + ua + wa * Y0[2];
+  C.a00 = wa * Y0[3];
+  C.a01 = wa * Y0[4];
+  r0 = f[m] - sa * XL2[0] - la * XL1[0] - wa * Y0[0];
+  // eq b: sb x_{m-1} + lb t0 + db t1 + ub y0 + wb y1 = f_{m+1}
+  A.a10 = sb * XL1[1];
+  A.a11 = sb * XL1[2];
+  B.a10 = sb * XL1[3] + lb + ub * Y0[1] + wb * Y1[1];
+  B.a11 = sb * XL1[4] + db + ub * Y0[2] + wb * Y1[2];
+  C.a10 = ub * Y0[3] + wb * Y1[3];
+  C.a11 = ub * Y0[4] + wb * Y1[4];
+  r1 = f[m + 1] - sb * XL1[0] - ub * Y0[0] - wb * Y1[0];
+}
+
+// Solve the chunk owned by this lane.  On entry s,l,d,u,w,f hold rows
+// r0..r0+M-1 of the five diagonals and the RHS; on return f holds the
+// solution of those rows.  p is the chunk index, P the chunk count.
+template <typename T, int M, typename Ex>
+__device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
+                                           const T (&d)[M], const T (&u)[M],
+                                           const T (&w)[M], T (&f)[M], int p,
+                                           int P, Ex &ex) {
+  static_assert(M >= 4, "a chunk needs at least two interior rows");
+  constexpr int m = M - 2; // interior rows 0..m-1, separator rows m, m+1
+  const bool first = (p == 0);
+  const bool last = (p == P - 1);
+
+  T du2[m], dw2[m], zf[m], za[m], zb[m];
+  T XL1[5], XL2[5], XF0[5], XF1[5];
+  chunk_eliminate<T, M>(s, l, d, u, w, f, first, du2, dw2, zf, za, zb, XL1, XL2, XF0, XF1);
 
   // ---- 3. separator equations -> one 2x2 block row per lane --------------
   T Y0[5], Y1[5]; // next chunk's interior rows 0, 1 (zeros for the last)
@@ -224,30 +322,10 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
   }
   // separator rows m (eq a) and m+1 (eq b); couplings past the end of the
   // system are out of range for the last chunk.
-  const T sa = s[m], la = l[m], da = d[m], ua = u[m];
-  const T wa = last ? T(0) : w[m];
-  const T sb = s[m + 1], lb = l[m + 1], db = d[m + 1];
-  const T ub = last ? T(0) : u[m + 1];
-  const T wb = last ? T(0) : w[m + 1];
-
   M2<T> A, B, C;
   T r0, r1;
-  // eq a: sa x_{m-2} + la x_{m-1} + da t0 + ua t1 + wa y0 = f_m
-  A.a00 = sa * XL2[1] + la * XL1[1];
-  A.a01 = sa * XL2[2] + la * XL1[2];
-  B.a00 = sa * XL2[3] + la * XL1[3] + da + wa * Y0[1];
-  B.a01 = sa * XL2[4] + la * XL1[4] + ua + wa * Y0[2];
-  C.a00 = wa * Y0[3];
-  C.a01 = wa * Y0[4];
-  r0 = f[m] - sa * XL2[0] - la * XL1[0] - wa * Y0[0];
-  // eq b: sb x_{m-1} + lb t0 + db t1 + ub y0 + wb y1 = f_{m+1}
-  A.a10 = sb * XL1[1];
-  A.a11 = sb * XL1[2];
-  B.a10 = sb * XL1[3] + lb + ub * Y0[1] + wb * Y1[1];
-  B.a11 = sb * XL1[4] + db + ub * Y0[2] + wb * Y1[2];
-  C.a10 = ub * Y0[3] + wb * Y1[3];
-  C.a11 = ub * Y0[4] + wb * Y1[4];
-  r1 = f[m + 1] - sb * XL1[0] - ub * Y0[0] - wb * Y1[0];
+  separator_block_row<T, M>(s, l, d, u, f, XL1, XL2, Y0, Y1, last ? T(0) : w[m],
+                            last ? T(0) : u[m + 1], last ? T(0) : w[m + 1], A, B, C, r0, r1);
 
   // normalise: S_p + A S_{p-1} + C S_{p+1} = r
   {
@@ -272,17 +350,7 @@ __device__ __forceinline__ void part_solve(const T (&s)[M], const T (&l)[M],
     s0 = rp[0];
     s1 = rp[1];
   }
-  T xa = t0, xb = t1; // x_{i+1}, x_{i+2}
-#pragma unroll
-  for (int i = m - 1; i >= 0; --i) {
-    const T hi = zf[i] + za[i] * s0 + zb[i] * s1;
-    const T xi = hi - du2[i] * xa - dw2[i] * xb;
-    f[i] = xi;
-    xb = xa;
-    xa = xi;
-  }
-  f[m] = t0;
-  f[m + 1] = t1;
+  chunk_backsub<T, M>(du2, dw2, zf, za, zb, s0, s1, t0, t1, f);
 }
 
 } // namespace pentad

@@ -164,9 +164,40 @@ bool plan_x_long(const Lines &ln, int elem_bytes, Plan &pl) {
   return false;
 }
 
-Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const ptrs[6]) {
+// Lines shared by two workgroups (kernels_split.hip): chunks of 16 rows (8
+// for FP64 where 16 do not halve the line), S = 512 / P systems per line set
+// (32 or 64).  Automatic choice only for the outermost direction when a row
+// step is at least 1 MB (every row of a line in its own page) and there are
+// enough line sets to fill the chip; "thomas-pcr-split" by name wherever the
+// kernel applies.  Measured at 512^3 FP64 along z: 2.12 -> see DESIGN.
+bool plan_split(const Lines &ln, int elem_bytes, bool forced, Plan &pl) {
+  if (ln.dir == 0) return false;
+  if (!forced && (ln.dir != 2 || ln.len <= 384 || ln.stride * elem_bytes < (1 << 18) ||
+                  ln.n_sys < 8192))
+    return false;
+  for (int r : {16, 8}) {
+    if (ln.len % (2 * r) != 0) continue;
+    const int64_t P = ln.len / (2 * r);
+    if (P < 2 || P > 16) continue;
+    const int S = P <= 8 ? 64 : 32;
+    if (!part_yz_split_supported(elem_bytes, r, S, ln.len)) continue;
+    pl.kind = Plan::PartYZ;
+    pl.rows = r;
+    pl.sys_per_wg = S;
+    pl.split = true;
+    return true;
+  }
+  return false;
+}
+
+Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const ptrs[6],
+                bool allow_split) {
   Plan pl;
   std::string want = requested_algo(c, ln.dir);
+  // the two-workgroup strided kernel by name (an extension of the
+  // reference's names); where it does not apply, "thomas-pcr"
+  const bool want_split = want == "thomas-pcr-split";
+  if (want_split) want = "thomas-pcr";
   // x: an unrecognised name runs the automatic choice, as the reference's
   // PENTA_XALGO does (pentad_cuda.cu:1463-1488); along y / z any name other
   // than auto, thomas-pcr and shared-fact runs the one-lane kernel, as the
@@ -321,6 +352,9 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const p
 
   // strided directions
   if (want == "auto" || want == "thomas-pcr") {
+    if (allow_split && !(have_tuned && c->force_rows[ln.dir] > 0) &&
+        plan_split(ln, elem_bytes, want_split, pl))
+      return pl;
     if (have_tuned && tuned.kind == Plan::PartYZ &&
         part_yz_supported(elem_bytes, tuned.rows, tuned.sys_per_wg, ln.len)) {
       pl.kind = Plan::PartYZ;
@@ -394,7 +428,7 @@ Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const p
 
 // FP32 + refinement: partitioned kernels when the length splits into
 // supported chunks, else the one-lane-per-system kernel with scratch.
-Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
+Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6], bool allow_split) {
   Plan pl;
   pl.kind = Plan::IrLane;
   const std::string want = requested_algo(c, ln.dir);
@@ -492,8 +526,8 @@ Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]) {
   // exchanges separators through LDS.  Measured at 512^3: y 2.81 -> 1.15 ms,
   // z 3.54 -> 1.44 ms, solutions 4e-15 apart.  "thomas-pcr" asked for by
   // name keeps the refinement kernel.
-  if (want == "auto") {
-    Plan q = plan_solve(c, ln, 8, ptrs);
+  if (want == "auto" || want == "thomas-pcr-split") {
+    Plan q = plan_solve(c, ln, 8, ptrs, allow_split);
     if (q.kind == Plan::PartYZ) {
       q.kind = Plan::IrWideYZ;
       return q;
@@ -536,6 +570,10 @@ const char *plan_algo_name(const Plan &pl) {
 
 size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes) {
   if (pl.kind == Plan::IrLane) return ir_lane_scratch_bytes(ln);
+  if (pl.split && pl.kind == Plan::PartYZ)
+    return part_yz_split_scratch_bytes(ln, pl.sys_per_wg, elem_bytes);
+  if (pl.split && pl.kind == Plan::IrWideYZ) // an FP64 solve
+    return part_yz_split_scratch_bytes(ln, pl.sys_per_wg, 8);
   if (pl.kind == Plan::Naive)
     return (size_t)3 * (size_t)ln.n_sys * (size_t)ln.len * (size_t)elem_bytes;
   if (pl.kind == Plan::SfX || pl.kind == Plan::SfYZ)
@@ -551,8 +589,8 @@ std::string plan_config(const Plan &pl) {
                   pl.lanes);
     return buf;
   case Plan::PartYZ:
-    std::snprintf(buf, sizeof buf, "thomas-pcr rows=%d sys_per_wg=%d", pl.rows,
-                  pl.sys_per_wg);
+    std::snprintf(buf, sizeof buf, "thomas-pcr%s rows=%d sys_per_wg=%d",
+                  pl.split ? " split" : "", pl.rows, pl.sys_per_wg);
     return buf;
   case Plan::IrX:
     std::snprintf(buf, sizeof buf, "ir thomas-pcr rows=%d lanes=%d", pl.rows,
@@ -568,8 +606,8 @@ std::string plan_config(const Plan &pl) {
     std::snprintf(buf, sizeof buf, "ir transpose rows=%d lanes=%d", pl.rows, pl.lanes);
     return buf;
   case Plan::IrWideYZ:
-    std::snprintf(buf, sizeof buf, "ir widened rows=%d sys_per_wg=%d", pl.rows,
-                  pl.sys_per_wg);
+    std::snprintf(buf, sizeof buf, "ir widened%s rows=%d sys_per_wg=%d",
+                  pl.split ? " split" : "", pl.rows, pl.sys_per_wg);
     return buf;
   case Plan::IrWideX:
     std::snprintf(buf, sizeof buf, "ir widened rows=%d lanes=%d", pl.rows, pl.lanes);
@@ -598,6 +636,7 @@ hipError_t launch_plan(const Plan &pl, const Diags<T> &a, T *x, const Lines &ln,
   switch (pl.kind) {
   case Plan::PartX: return launch_part_x<T>(a, x, ln, pl.rows, pl.lanes, st);
   case Plan::PartYZ:
+    if (pl.split) return launch_part_yz_split<T>(a, x, ln, pl.rows, pl.sys_per_wg, buffer, st);
     return launch_part_yz<T>(a, x, ln, pl.rows, pl.sys_per_wg, st);
   case Plan::TrX: return launch_tr_x<T>(a, x, ln, pl.rows, pl.lanes, st);
   case Plan::Naive:
@@ -639,6 +678,8 @@ hipError_t launch_plan_ir(const Plan &pl, const Diags<float> &a, double *x,
   case Plan::IrTrX:
     return launch_ir_tr_x(a, x, ln, pl.rows, max_steps, tol, steps, st);
   case Plan::IrWideYZ: // no refinement sweep: *steps is left as it is
+    if (pl.split)
+      return launch_part_yz_split_widened(a, x, ln, pl.rows, pl.sys_per_wg, buffer, st);
     return launch_part_yz_widened(a, x, ln, pl.rows, pl.sys_per_wg, st);
   case Plan::IrWideX: // the same along x, in place
     return launch_part_x_map_widened(a, x, x, false, ln.n_sys, ln.len, xmap_inplace(ln),
@@ -673,7 +714,7 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
     return PENTAD_STATUS_INVALID_VALUE;
   }
   const void *ptrs[6] = {ds, dl, d, du, dw, x};
-  Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs);
+  Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs, true);
   const Diags<T> a{ds, dl, d, du, dw};
   hipStream_t st = c->stream;
   hipError_t err = hipSuccess;
@@ -707,6 +748,10 @@ pentadStatus_t gpsv_batch_pitched(Context *c, const T *ds, const T *dl, const T 
     return false;
   };
 
+  // the two-workgroup kernel without room for its trade: the one-workgroup
+  // plan (no scratch)
+  if (pl.split && !(buffer != nullptr && plan_scratch_bytes(pl, ln, (int)sizeof(T)) <= buffer_cap))
+    pl = plan_solve(c, ln, (int)sizeof(T), ptrs, false);
   const bool sf = pl.kind == Plan::SfX || pl.kind == Plan::SfYZ;
   if (!scratch_ok(pl)) return PENTAD_STATUS_INVALID_VALUE;
   if (pl.kind != Plan::Naive) err = launch_plan<T>(pl, a, x, ln, buffer, st);
@@ -757,11 +802,14 @@ pentadStatus_t gpsv_batch_ir(Context *c, const float *ds, const float *dl,
     return PENTAD_STATUS_INVALID_VALUE;
   }
   const void *ptrs[6] = {ds, dl, d, du, dw, x};
-  Plan pl = plan_ir(c, ln, ptrs);
+  Plan pl = plan_ir(c, ln, ptrs, true);
+  // the two-workgroup kernel without room for its trade: the one-workgroup plan
+  if (pl.split && buffer == nullptr) pl = plan_ir(c, ln, ptrs, false);
   const Diags<float> a{ds, dl, d, du, dw};
   hipError_t err = hipSuccess;
   if (pl.kind != Plan::IrLane)
-    err = launch_plan_ir(pl, a, x, ln, nullptr, max_steps, tol, steps, c->stream);
+    err = launch_plan_ir(pl, a, x, ln, pl.split ? buffer : nullptr, max_steps, tol, steps,
+                         c->stream);
   if (pl.kind != Plan::IrLane && err != hipSuccess) {
     if (debug_launch())
       std::fprintf(stderr, "[penta] %s declined: %s -- falling back\n",
@@ -810,15 +858,26 @@ template pentadStatus_t gpsv_batch<double>(Context *, const double *, const doub
                                            const double *, double *, const int *, int,
                                            int, void *);
 
+// cap: the scratch the caller will pass, when it is fixed (the two-workgroup
+// kernel's trade does not fit: the one-workgroup plan's need)
 template <typename T>
 size_t buffer_size(Context *c, const T *ds, const T *dl, const T *d,
                    const T *du, const T *dw, const T *x, const int *dims,
-                   int ndim, int solvedim, const int *pads = nullptr) {
+                   int ndim, int solvedim, const int *pads = nullptr,
+                   size_t cap = SIZE_MAX) {
   Lines ln;
   if (decompose_padded(dims, pads, ndim, solvedim, &ln) != 0) return 0;
   const void *ptrs[6] = {ds, dl, d, du, dw, x};
-  const Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs);
+  Plan pl = plan_solve(c, ln, (int)sizeof(T), ptrs, true);
+  if (pl.split && plan_scratch_bytes(pl, ln, (int)sizeof(T)) > cap)
+    pl = plan_solve(c, ln, (int)sizeof(T), ptrs, false);
   return plan_scratch_bytes(pl, ln, (int)sizeof(T));
+}
+
+size_t noalloc_need(Context *c, const double *ds, const double *dl, const double *d,
+                    const double *du, const double *dw, const double *x, const int *dims,
+                    int ndim, int solvedim, size_t cap) {
+  return buffer_size<double>(c, ds, dl, d, du, dw, x, dims, ndim, solvedim, nullptr, cap);
 }
 
 } // namespace pentad
@@ -936,7 +995,7 @@ size_t pentadsolver_DS_gpsv_batch_ir_buffer_size_ext(
   Lines ln;
   if (decompose(t_dims, t_ndim, t_solvedim, &ln) != 0) return 0;
   const void *ptrs[6] = {ds, dl, d, du, dw, x};
-  return plan_scratch_bytes(plan_ir(ctx_of(h), ln, ptrs), ln, 4);
+  return plan_scratch_bytes(plan_ir(ctx_of(h), ln, ptrs, true), ln, 4);
 }
 
 void pentadsolver_DS_gpsv_batch_ir(pentadsolver_handle_t h, const float *ds,
@@ -968,7 +1027,7 @@ pentadStatus_t pentadsolver_set_algo(pentadsolver_handle_t h, int dir,
   std::string s = algo ? algo : "";
   if (s == "legacy") s = "naive";
   if (!s.empty() && s != "auto" && s != "naive" && s != "transpose" &&
-      s != "thomas-pcr" && s != "shared-fact")
+      s != "thomas-pcr" && s != "thomas-pcr-split" && s != "shared-fact")
     return PENTAD_STATUS_INVALID_VALUE;
   std::lock_guard<std::mutex> lk(c->mu);
   c->algo[dir] = s;

@@ -20,6 +20,7 @@ struct Plan {
   int rows = 0;       // rows per lane (partitioned kernels)
   int lanes = 0;      // lanes per system (x)
   int sys_per_wg = 0; // systems per workgroup (y/z)
+  bool split = false; // PartYZ / IrWideYZ: each line shared by two workgroups
 };
 
 struct TuneKey {
@@ -59,9 +60,18 @@ std::string requested_algo(Context *c, int dir);
 std::string env_algo(int dir);
 bool debug_launch();
 
-Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const ptrs[6]);
-Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6]);
+// allow_split: the plan may share the lines of the outermost direction
+// between two workgroups (scratch for their trade in the caller's buffer);
+// the batched solve asks again without it when the caller passes none.
+Plan plan_solve(Context *c, const Lines &ln, int elem_bytes, const void *const ptrs[6],
+                bool allow_split = false);
+Plan plan_ir(Context *c, const Lines &ln, const void *const ptrs[6], bool allow_split = false);
 size_t plan_scratch_bytes(const Plan &pl, const Lines &ln, int elem_bytes);
+// scratch of the FP64 batched solve when the caller's buffer holds cap bytes
+// (the allocation-free entry point's fixed array)
+size_t noalloc_need(Context *c, const double *ds, const double *dl, const double *d,
+                    const double *du, const double *dw, const double *x, const int *dims,
+                    int ndim, int solvedim, size_t cap);
 template <typename T>
 hipError_t launch_plan(const Plan &pl, const Diags<T> &a, T *x, const Lines &ln,
                        void *buffer, hipStream_t st);

@@ -130,6 +130,17 @@ template <typename T>
 hipError_t launch_part_yz(const Diags<T> &a, T *x, const Lines &ln, int rows,
                           int sys_per_wg, hipStream_t st);
 bool part_yz_supported(int elem_bytes, int rows, int sys_per_wg, int64_t len);
+// The same with each line shared by two workgroups (the first and second
+// half of its rows; kernels_split.hip): S systems per line set, scratch of
+// part_yz_split_scratch_bytes in the caller's buffer.  For the outermost
+// direction, where every row of a line lies in its own page.
+bool part_yz_split_supported(int elem_bytes, int rows, int S, int64_t len);
+size_t part_yz_split_scratch_bytes(const Lines &ln, int S, int elem_bytes);
+template <typename T>
+hipError_t launch_part_yz_split(const Diags<T> &a, T *x, const Lines &ln, int rows, int S,
+                                void *scratch, hipStream_t st);
+hipError_t launch_part_yz_split_widened(const Diags<float> &a, double *x, const Lines &ln,
+                                        int rows, int S, void *scratch, hipStream_t st);
 
 // LDS-staged x solve for any length (rows 8 or 16, lanes a power of two,
 // len <= rows * lanes; rows past the end are identity rows): "transpose".

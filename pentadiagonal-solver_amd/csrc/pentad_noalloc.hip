@@ -5,7 +5,8 @@
 // pentadsolver_D_gpsv_batch_noalloc runs exactly the solve of
 // pentadsolver_D_gpsv_batch (same planner, kernels and errors) with its
 // scratch -- the one-lane kernel's [3][len][n_sys] intermediates, the
-// shared factorisation's factor table -- in `scratch` below, a fixed array
+// shared factorisation's factor table, the two-workgroup kernel's ticket
+// counters, flags and trade words -- in `scratch` below, a fixed array
 // in the library's own device memory, instead of a caller buffer.  Every
 // GPU the library runs on holds its own copy of the array until the process
 // ends.  A solve whose scratch need exceeds PENTAD_NOALLOC_SCRATCH_BYTES is
@@ -74,8 +75,10 @@ extern "C" pentadStatus_t pentadsolver_D_gpsv_batch_noalloc(
     set_status(c, PENTAD_STATUS_EXECUTION_FAILED);
     return PENTAD_STATUS_EXECUTION_FAILED;
   }
-  const size_t need = pentadsolver_D_gpsv_batch_buffer_size_ext(h, ds, dl, d, du, dw, x, t_dims,
-                                                                t_ndim, t_solvedim);
+  // (a plan whose lines are shared by two workgroups runs on one workgroup
+  // per line when its trade does not fit the array)
+  const size_t need = noalloc_need(c, ds, dl, d, du, dw, x, t_dims, t_ndim, t_solvedim,
+                                   PENTAD_NOALLOC_SCRATCH_BYTES);
   if (need > PENTAD_NOALLOC_SCRATCH_BYTES) {
     std::fprintf(stderr,
                  "[penta] noalloc: the solve along solvedim %d needs %zu bytes of scratch, more "

@@ -115,3 +115,33 @@ def test_noalloc_null_handle():
     torch.cuda.synchronize()
     ref = O.solve(*coef, b.copy(), dims, 2)
     assert float(np.max(np.abs(x.cpu().numpy() - ref)) / np.max(np.abs(ref))) < 1e-12
+
+
+def test_noalloc_two_workgroup_lines():
+    """Outermost lines of 512 rows, 256 KB apart: the automatic choice shares
+    each line between two workgroups, its trade words in the fixed array when
+    they fit (3 MiB here); on a 512^3 field they do not (24 MiB), and the
+    solve runs one workgroup per line instead of being refused."""
+    with P.Handle() as h:
+        for dims, split in (([256, 128, 512], True), ([512, 512, 512], False)):
+            print(f"noalloc split dims={dims}", flush=True)
+            coef, b = gen(dims, 17)
+            t = [torch.from_numpy(a).cuda() for a in coef]
+            xb = torch.from_numpy(b.copy()).cuda()
+            if split:
+                P.solve(*t, xb, dims, 2, handle=h)  # caller buffer: the split kernel
+            else:
+                P.gpsv_batch(*t, xb, dims, 2, None, h)  # no buffer: one workgroup per line
+            cfg_b = h.kernel_config(2)
+            xn = torch.from_numpy(b.copy()).cuda()
+            assert noalloc(t, xn, dims, 2, h) == 0
+            torch.cuda.synchronize()
+            cfg_n = h.kernel_config(2)
+            assert cfg_n == cfg_b and cfg_n.startswith("thomas-pcr split") == split, cfg_n
+            assert torch.equal(xn, xb)
+            if split:
+                ref = O.solve(*coef, b.copy(), dims, 2)
+                err = float(np.max(np.abs(xn.cpu().numpy() - ref)) / np.max(np.abs(ref)))
+                assert err < 1e-12, err
+            del t, xb, xn
+            torch.cuda.empty_cache()
