@@ -186,7 +186,12 @@ pentadStatus_t pentadsolver_set_stream(pentadsolver_handle_t handle,
  * the reference's PENTA_XALGO / PENTA_YALGO / PENTA_ZALGO environment
  * variables (src/cuda/singlenode/pentad_cuda.cu:1455-1489, :1163-1174):
  * "auto", "naive", "transpose", "thomas-pcr", "shared-fact" (and "legacy"
- * for y/z, an alias of "naive").  A handle setting wins over the
+ * for y/z, an alias of "naive"), plus one name of this library's own,
+ * "thomas-pcr-split": along y / z each line shared by two workgroups (the
+ * automatic choice for the outermost direction of lines over 384 rows whose
+ * rows lie 256 KB or more apart, when the caller passes the scratch that
+ * pentadsolver_*_gpsv_batch_buffer_size_ext reports; without it the solve
+ * runs one workgroup per line).  A handle setting wins over the
  * environment; NULL or "" clears it.  The environment is read on every call
  * unless pentadsolver_freeze_env(handle, 1) was called. */
 pentadStatus_t pentadsolver_set_algo(pentadsolver_handle_t handle, int dir,
@@ -569,7 +574,9 @@ pentadStatus_t pentadsolver_S_dist_backward(pentadsolver_handle_t handle, const 
                                             void *t_buffer);
 
 /* Allocation-free FP64 solve: pentadsolver_D_gpsv_batch (same kernels,
- * same results bit for bit, same errors) with the scratch taken from a fixed
+ * same results bit for bit, same errors -- except outermost lines whose
+ * two-workgroup trade does not fit the array, which run one workgroup per
+ * line as with a NULL t_buffer) with the scratch taken from a fixed
  * array in the library's own device memory instead of t_buffer, so a call
  * allocates nothing and can be recorded under HIP graph capture.  No
  * reference counterpart: it is for callers that cannot allocate (code under
@@ -577,7 +584,9 @@ pentadStatus_t pentadsolver_S_dist_backward(pentadsolver_handle_t handle, const 
  * Limit: a solve whose scratch need (what
  * pentadsolver_D_gpsv_batch_buffer_size_ext reports; 0 for the on-chip
  * kernels, 24 B per row for the one-lane kernel, the factor table for
- * shared-fact) exceeds PENTAD_NOALLOC_SCRATCH_BYTES is refused with
+ * shared-fact; the two-workgroup kernel's trade words, about 3 B per row,
+ * when they fit the array, else the one-workgroup kernel runs and needs
+ * none) exceeds PENTAD_NOALLOC_SCRATCH_BYTES is refused with
  * PENTAD_STATUS_INVALID_VALUE and a message on stderr, and nothing is
  * launched.  Every GPU the library runs on holds one such array until the
  * process ends (the device address is looked up at pentadsolver_create, or
