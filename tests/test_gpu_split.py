@@ -109,3 +109,32 @@ def test_split_widened_mixed_precision():
     assert int(buf[:256].view(torch.int32)[TIMEOUT_WORD].item()) == 0
     ref = oracle_solve(arrs[:5] + [b], dims, 2)
     assert relerr(x.cpu().numpy(), ref) < FP64_TOL
+
+
+def test_split_graph_replay():
+    """The two-workgroup solve captured into a hipGraph (the reset is a
+    kernel node, not a memset): every replay, with eager work and
+    synchronisations between them, matches the eager solve bit for bit and
+    leaves the time-out word clear."""
+    dims = [33, 7, 512]
+    arrs = gen(dims, 19, np.float64)
+    t = [dev(a) for a in arrs]
+    eager, _, nb, _ = split_solve(arrs, dims, 2)
+    s = torch.cuda.Stream()
+    with P.Handle(stream=s) as h:
+        h.set_algo(2, "thomas-pcr-split")
+        buf = torch.zeros(nb, dtype=torch.uint8, device="cuda")
+        x = t[5].clone()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            P.gpsv_batch(*t[:5], x, dims, 2, buf, h, check=False)
+        assert h.kernel_config(2) == "thomas-pcr split rows=16 sys_per_wg=32"
+        for rep in range(3):
+            x.copy_(t[5])
+            buf[:256].fill_(7)  # stale header words: the reset node must clear them
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(x.cpu().numpy(), eager), rep
+            assert int(buf[:256].view(torch.int32)[TIMEOUT_WORD].item()) == 0
